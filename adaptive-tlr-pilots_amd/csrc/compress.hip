@@ -1,0 +1,414 @@
+// Generation and initial compression of the Matern-3/2 covariance tiles.
+//
+// Reference semantics (codes/17_tlr_cholesky_pilot.py:155-166, 107-116): every off-diagonal tile
+// T = sigma2 * matern32(dist(S_i, S_j), r) is truncated to the smallest rank r with
+// ||T - T_r||_F <= thr (thr = eps ||T||_F for the fixed rule, tau b sigma2 eta / N for the adaptive one),
+// stored as (U = U_r s_r, V = V_r).
+//
+// MI355X design: T is never stored.  Its entries are regenerated from the 2 x b site coordinates in
+// every product pass (an exp and a sqrt per entry, no HBM traffic).  The subspace comes from alternating
+// products with T and T^T (a randomized range finder with power iteration, Halko-Martinsson-Tropp), each
+// followed by a register-resident Householder QR.  The rank decision uses an FP64-verified tail:
+//   tail[r]^2 = ||T - T P||_F^2 (computed directly, entry by entry) + sum_{i>=r} sigma_i(core)^2,
+// which differs from the exact SVD tail only by the subspace defect; a tile whose rank leaves less than
+// `pmin` spare sketch columns is redone with a wider sketch.
+#include "common.hpp"
+#include "panel.hpp"
+#include "tlr_kernels.hpp"
+
+namespace tlr {
+
+// ------------------------------------------------------------------------------------------ diagonal
+// D_i = sigma2 matern32(S_i, S_i) + sigma2 eta I, padded to b x b with the identity beyond the tile.
+__global__ void k_gen_diag(const double* __restrict__ sx, const double* __restrict__ sy, int64_t n, int b,
+                           MaternParams mp, double* __restrict__ D) {
+  const int i = blockIdx.y;
+  const int64_t i0 = (int64_t)i * b;
+  const int bi = (int)min<int64_t>(b, n - i0);
+  double* Di = D + (int64_t)i * b * b;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)b * b;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int p = (int)(e % b), q = (int)(e / b);
+    double v;
+    if (p < bi && q < bi) {
+      v = matern32(mp, sx[i0 + p], sy[i0 + p], sx[i0 + q], sy[i0 + q]);
+      if (p == q) v += mp.nugget;
+    } else {
+      v = (p == q) ? 1.0 : 0.0;
+    }
+    Di[e] = v;
+  }
+}
+
+// --------------------------------------------------------------------------------- Matern products
+// out_t (b x k) = op(T_t) X_t for every active tile t.  op = 0: T (rows: sites of tile i, inner index:
+// sites of tile j); op = 1: T^T (the roles of the two site sets swap).  X_t has leading dimension b;
+// xstride = 0 shares one X (the Gaussian test matrix) across tiles.  Work group: 256 threads computing a
+// 256-row x 64-column block as 8 x 8 register blocks; the inner index runs in chunks of 16 whose kernel
+// entries are generated once into LDS.
+constexpr int MP_ROWS = 256, MP_COLS = 64, MP_QC = 16;
+
+__global__ __launch_bounds__(256) void k_matern_product(const double* __restrict__ sx, const double* __restrict__ sy,
+                                                        int64_t n, int b, MaternParams mp, int op,
+                                                        const double* __restrict__ X, int64_t xstride,
+                                                        double* __restrict__ out, int64_t ostride, int k,
+                                                        const int* __restrict__ active) {
+  const int64_t t = blockIdx.z;
+  if (active && !active[t]) return;
+  int i, j;
+  tri_decode(t, &i, &j);
+  if (op) { const int x = i; i = j; j = x; }
+  const int64_t r0g = (int64_t)i * b, q0g = (int64_t)j * b;
+  const int mr = (int)min<int64_t>(b, n - r0g), mq = (int)min<int64_t>(b, n - q0g);
+  const int rb = blockIdx.x * MP_ROWS, cb = blockIdx.y * MP_COLS;
+  __shared__ double Ts[MP_QC][MP_ROWS];
+  __shared__ double Xs[MP_QC][MP_COLS];
+  __shared__ double rx[MP_ROWS], ry[MP_ROWS];
+  __shared__ double qx[MP_QC], qy[MP_QC];
+  const int tid = threadIdx.x;
+  const int tr = tid / 8, tc = tid % 8;    // 32 row groups x 8 column groups
+  {
+    const int p = rb + tid;
+    rx[tid] = (p < mr) ? sx[r0g + p] : 0.0;
+    ry[tid] = (p < mr) ? sy[r0g + p] : 0.0;
+  }
+  const double* Xt = X + t * xstride;
+  double acc[8][8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[a][c] = 0.0;
+  __syncthreads();
+  for (int q0 = 0; q0 < mq; q0 += MP_QC) {
+    if (tid < MP_QC) {
+      const int q = q0 + tid;
+      qx[tid] = (q < mq) ? sx[q0g + q] : 0.0;
+      qy[tid] = (q < mq) ? sy[q0g + q] : 0.0;
+    }
+    __syncthreads();
+    // kernel entries: 256 rows x 16 inner indices; thread tid generates row tid
+    {
+      const bool rowok = rb + tid < mr;
+      const double px = rx[tid], py = ry[tid];
+#pragma unroll 4
+      for (int qq = 0; qq < MP_QC; ++qq) {
+        double v = 0.0;
+        if (rowok && q0 + qq < mq) v = matern32(mp, px, py, qx[qq], qy[qq]);
+        Ts[qq][tid] = v;
+      }
+    }
+    for (int e = tid; e < MP_QC * MP_COLS; e += 256) {
+      const int qq = e % MP_QC, cc = e / MP_QC;
+      const int q = q0 + qq, c = cb + cc;
+      Xs[qq][cc] = (q < mq && c < k) ? Xt[q + (int64_t)c * b] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int qq = 0; qq < MP_QC; ++qq) {
+      double tv[8], xv[8];
+#pragma unroll
+      for (int a = 0; a < 8; ++a) tv[a] = Ts[qq][tr * 8 + a];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) xv[c] = Xs[qq][tc * 8 + c];
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[a][c] = fma(tv[a], xv[c], acc[a][c]);
+    }
+    __syncthreads();
+  }
+  double* Ot = out + t * ostride;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int p = rb + tr * 8 + a;
+    if (p < b) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int cc = cb + tc * 8 + c;
+        if (cc < k) Ot[p + (int64_t)cc * b] = (p < mr) ? acc[a][c] : 0.0;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------ orthonormalisation
+// In place: M_t (b x k) -> Q_t (explicit, b x k) of its Householder QR; optionally R_t (k x k, col-major)
+// stored to R (transposed when r_transposed).
+template <int RPL, int NW>
+__global__ __launch_bounds__(64 * NW) void k_panel_qr(double* __restrict__ M, int64_t mstride, int b, int k,
+                                                        double* __restrict__ R, int64_t rstride, int r_transposed,
+                                                        const int* __restrict__ active) {
+  const int64_t t = blockIdx.x;
+  if (active && !active[t]) return;
+  __shared__ double vbuf[2 * 64 * RPL];
+  __shared__ double tau[8 * NW];
+  RegPanel<RPL, NW> P;
+  double* Mt = M + t * mstride;
+  P.load(Mt, b, b, k);
+  P.geqr2(b, k, vbuf, tau);
+  if (R) {
+    if (r_transposed) P.store_rt(R + t * rstride, k, k);
+    else P.store_r(R + t * rstride, k, k);
+  }
+  P.org2r(b, k, vbuf, tau);
+  P.store(Mt, b, b, k);
+}
+
+// ------------------------------------------------------------------------------------ residual
+// For tile t with T ~ A B^T (A: b x k over the row sites of tile i, B: b x k over the sites of tile j),
+// partial sums over 64 x 64 blocks of ||T - A B^T||_F^2 and ||T||_F^2, written to
+// part[(t * nblk + blk) * 2 + {0, 1}].  Deterministic: the rank kernel adds them in block order.
+__global__ __launch_bounds__(256) void k_residual(const double* __restrict__ sx, const double* __restrict__ sy,
+                                                  int64_t n, int b, MaternParams mp,
+                                                  const double* __restrict__ A, const double* __restrict__ B,
+                                                  int64_t stride, int k, double* __restrict__ part,
+                                                  const int* __restrict__ active) {
+  const int64_t t = blockIdx.z;
+  if (active && !active[t]) return;
+  int i, j;
+  tri_decode(t, &i, &j);
+  const int64_t r0g = (int64_t)i * b, q0g = (int64_t)j * b;
+  const int mr = (int)min<int64_t>(b, n - r0g), mq = (int)min<int64_t>(b, n - q0g);
+  const int nbb = (b + 63) / 64;
+  const int pb = blockIdx.x * 64, qb = blockIdx.y * 64;
+  __shared__ double As[32][64];
+  __shared__ double Bs[32][64];
+  __shared__ double red[2][4];
+  const int tid = threadIdx.x;
+  const int tr = tid / 16, tc = tid % 16;   // 16 x 16 groups of 4 x 4
+  const double* At = A + t * stride;
+  const double* Bt = B + t * stride;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+  for (int c0 = 0; c0 < k; c0 += 32) {
+    for (int e = tid; e < 32 * 64; e += 256) {
+      const int rr = e % 64, cc = e / 64;
+      const int c = c0 + cc;
+      As[cc][rr] = (c < k && pb + rr < b) ? At[pb + rr + (int64_t)c * b] : 0.0;
+      Bs[cc][rr] = (c < k && qb + rr < b) ? Bt[qb + rr + (int64_t)c * b] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int cc = 0; cc < 32; ++cc) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = As[cc][tr * 4 + a];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bv[c] = Bs[cc][tc * 4 + c];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] = fma(av[a], bv[c], acc[a][c]);
+    }
+    __syncthreads();
+  }
+  double res2 = 0.0, fro2 = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int p = pb + tr * 4 + a;
+    if (p >= mr) continue;
+    const double px = sx[r0g + p], py = sy[r0g + p];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int q = qb + tc * 4 + c;
+      if (q >= mq) continue;
+      const double tv = matern32(mp, px, py, sx[q0g + q], sy[q0g + q]);
+      const double d = tv - acc[a][c];
+      res2 += d * d;
+      fro2 += tv * tv;
+    }
+  }
+  res2 = wave_sum(res2);
+  fro2 = wave_sum(fro2);
+  if (lane_id() == 0) { red[0][wave_id()] = res2; red[1][wave_id()] = fro2; }
+  __syncthreads();
+  if (tid == 0) {
+    const int64_t blk = (int64_t)blockIdx.y * nbb + blockIdx.x;
+    part[(t * nbb * nbb + blk) * 2 + 0] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+    part[(t * nbb * nbb + blk) * 2 + 1] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
+  }
+}
+
+// ------------------------------------------------------------------------------ rank decision
+// For tile t: core = R (k x k, already transposed by the QR kernel when the last product was with T^T),
+// one-sided Jacobi core J = G; tails from the verified residual; rank by the rule; outputs
+// U = Qrow G[:, perm[:r]], V = Qcol J[:, perm[:r]] into the factor slots.
+// need[t] <- 0 when accepted, 1 when a wider sketch is needed.  status[0] (rank cap) is raised with
+// an atomic max when the rank exceeds the slot capacity.
+__global__ __launch_bounds__(256) void k_rank_decide(const double* __restrict__ Rm, int64_t rstride,
+                                                     const double* __restrict__ Qrow, const double* __restrict__ Qcol,
+                                                     int64_t qstride, const double* __restrict__ part, int b, int k,
+                                                     int kmax, int pmin, int rule_fixed, double thr_rel, double thr_abs,
+                                                     double* __restrict__ U, double* __restrict__ V, int64_t fstride,
+                                                     int rank_cap, int* __restrict__ rank, int* __restrict__ need,
+                                                     double* __restrict__ tail_out, int* __restrict__ status) {
+  const int64_t t = blockIdx.x;
+  if (!need[t]) return;
+  extern __shared__ double lds[];
+  double* G = lds;                // k x k
+  double* J = G + k * k;          // k x k
+  double* sig = J + k * k;        // k
+  double* tail = sig + k;         // k + 1
+  int* perm = (int*)(tail + k + 1);
+  __shared__ int flag;
+  __shared__ int s_rank;
+  __shared__ double s_res2, s_fro2;
+  const int tid = threadIdx.x;
+  const double* Rt = Rm + t * rstride;
+  for (int e = tid; e < k * k; e += blockDim.x) G[e] = Rt[e];
+  if (tid == 0) {
+    const int nbb = (b + 63) / 64;
+    double r2 = 0.0, f2 = 0.0;
+    for (int e = 0; e < nbb * nbb; ++e) {
+      r2 += part[(t * nbb * nbb + e) * 2 + 0];
+      f2 += part[(t * nbb * nbb + e) * 2 + 1];
+    }
+    s_res2 = r2; s_fro2 = f2;
+  }
+  __syncthreads();
+  jacobi_svd<4>(G, k, J, k, k, sig, perm, &flag, 60);
+  if (tid == 0) {
+    // tail[r] = sqrt(res2 + sum_{i >= r} sig_perm[i]^2), r = 0..k
+    double acc = s_res2;
+    tail[k] = sqrt(acc);
+    for (int r = k - 1; r >= 0; --r) {
+      const double s = sig[perm[r]];
+      acc += s * s;
+      tail[r] = sqrt(acc);
+    }
+    const double thr = rule_fixed ? thr_rel * sqrt(s_fro2) : thr_abs;
+    int r = -1;
+    for (int c = 0; c <= k; ++c)
+      if (tail[c] <= thr) { r = c; break; }
+    int accept;
+    if (r < 0) accept = 0;                              // residual alone exceeds the budget
+    else accept = (r + pmin <= k) || (k >= kmax);
+    if (r < 0 && k >= kmax) { atomicMax(&status[0], 1); r = k; accept = 1; }
+    if (accept && r > rank_cap) { atomicMax(&status[0], 1); r = rank_cap; }
+    s_rank = accept ? r : -1;
+    if (tail_out) tail_out[t] = (r >= 0 && r <= k) ? tail[r] : -1.0;
+  }
+  __syncthreads();
+  const int r = s_rank;
+  if (r < 0) {
+    if (tid == 0) need[t] = 1;
+    return;
+  }
+  const double* Qr = Qrow + t * qstride;
+  const double* Qc = Qcol + t * qstride;
+  double* Ut = U + t * fstride;
+  double* Vt = V + t * fstride;
+  for (int e = tid; e < b * r; e += blockDim.x) {
+    const int p = e % b, c = e / b;
+    const int pc = perm[c];
+    double u = 0.0, v = 0.0;
+    for (int l = 0; l < k; ++l) {
+      u = fma(Qr[p + (int64_t)l * b], G[l + pc * k], u);
+      v = fma(Qc[p + (int64_t)l * b], J[l + pc * k], v);
+    }
+    Ut[p + (int64_t)c * b] = u;
+    Vt[p + (int64_t)c * b] = v;
+  }
+  if (tid == 0) { rank[t] = r; need[t] = 0; }
+}
+
+__global__ void k_fill_int(int* p, int64_t count, int value) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = value;
+}
+
+__global__ void k_copy_int(int* dst, const int* src, int64_t count) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x)
+    dst[e] = src[e];
+}
+
+// ------------------------------------------------------------------------------------ launchers
+static int nblk(int64_t a, int64_t d) { return (int)((a + d - 1) / d); }
+
+hipError_t launch_fill_int(int* p, int64_t count, int value, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_int, dim3(min(nblk(count, 256), 1024)), dim3(256), 0, s, p, count, value);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_int, dim3(min(nblk(count, 256), 1024)), dim3(256), 0, s, dst, src, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen_diag(const double* sx, const double* sy, int64_t n, int b, int nB, MaternParams mp,
+                           double* D, hipStream_t s) {
+  dim3 grid(nblk((int64_t)b * b, 256 * 8), nB);
+  hipLaunchKernelGGL(k_gen_diag, grid, dim3(256), 0, s, sx, sy, n, b, mp, D);
+  return hipGetLastError();
+}
+
+hipError_t launch_matern_product(const double* sx, const double* sy, int64_t n, int b, int64_t ntile, MaternParams mp,
+                                 int op, const double* X, int64_t xstride, double* out, int64_t ostride, int k,
+                                 const int* active, hipStream_t s) {
+  dim3 grid(nblk(b, MP_ROWS), nblk(k, MP_COLS), (unsigned)ntile);
+  hipLaunchKernelGGL(k_matern_product, grid, dim3(256), 0, s, sx, sy, n, b, mp, op, X, xstride, out, ostride, k,
+                     active);
+  return hipGetLastError();
+}
+
+template <int RPL, int NW>
+static void launch_qr_t(double* M, int64_t mstride, int b, int k, double* R, int64_t rstride, int rt,
+                        const int* active, int64_t ntile, hipStream_t s) {
+  hipLaunchKernelGGL((k_panel_qr<RPL, NW>), dim3((unsigned)ntile), dim3(64 * NW), 0, s, M, mstride, b, k, R, rstride,
+                     rt, active);
+}
+
+hipError_t launch_panel_qr(double* M, int64_t mstride, int b, int k, double* R, int64_t rstride, int r_transposed,
+                           const int* active, int64_t ntile, hipStream_t s) {
+  if (b > 512 || k > 128 || k > b) return hipErrorInvalidValue;
+  const int nw = (k + 7) / 8;
+  if (b <= 256) {
+    if (nw <= 4) launch_qr_t<4, 4>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
+    else if (nw <= 8) launch_qr_t<4, 8>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
+    else if (nw <= 12) launch_qr_t<4, 12>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
+    else launch_qr_t<4, 16>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
+  } else {
+    if (nw <= 4) launch_qr_t<8, 4>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
+    else if (nw <= 8) launch_qr_t<8, 8>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
+    else if (nw <= 12) launch_qr_t<8, 12>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
+    else launch_qr_t<8, 16>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_residual(const double* sx, const double* sy, int64_t n, int b, int64_t ntile, MaternParams mp,
+                           const double* A, const double* B, int64_t stride, int k, double* part, const int* active,
+                           hipStream_t s) {
+  const int nbb = nblk(b, 64);
+  dim3 grid(nbb, nbb, (unsigned)ntile);
+  hipLaunchKernelGGL(k_residual, grid, dim3(256), 0, s, sx, sy, n, b, mp, A, B, stride, k, part, active);
+  return hipGetLastError();
+}
+
+size_t rank_decide_lds(int k) {
+  return (size_t)(2 * k * k + 2 * k + 1) * sizeof(double) + (size_t)k * sizeof(int);
+}
+
+hipError_t launch_rank_decide(const double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
+                              int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
+                              double thr_rel, double thr_abs, double* U, double* V, int64_t fstride, int rank_cap,
+                              int* rank, int* need, double* tail_out, int* status, int64_t ntile, hipStream_t s) {
+  const size_t lds = rank_decide_lds(k);
+  if (lds > 150 * 1024) return hipErrorInvalidValue;
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)k_rank_decide, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_rank_decide, dim3((unsigned)ntile), dim3(256), lds, s, Rm, rstride, Qrow, Qcol, qstride, part,
+                     b, k, kmax, pmin, rule_fixed, thr_rel, thr_abs, U, V, fstride, rank_cap, rank, need, tail_out,
+                     status);
+  return hipGetLastError();
+}
+
+}  // namespace tlr

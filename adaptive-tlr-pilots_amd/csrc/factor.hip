@@ -1,0 +1,571 @@
+// Right-looking TLR Cholesky on one GPU, one launch per phase of step k.
+//
+// Reference semantics (codes/17_tlr_cholesky_pilot.py:171-205, 230-240, 259-266):
+//   L_kk = chol(A_kk)                         (non-PD pivot -> completed=False, failed_at=k)
+//   V_ik <- L_kk^{-1} V_ik, i > k             (panel; U_ik unchanged)
+//   A_ii -= (U_ik W) U_ik^T, W = V_ik^T V_ik  (diagonal low-rank update)
+//   (U_ij, V_ij) <- recompress([U_ij, -U_ik W], [V_ij, U_jk]), W = V_ik^T V_jk, k < j < i
+//   z = L^{-1} y by tile forward substitution; log L = -(z^T z + 2 sum log diag L_kk + N log 2 pi)/2.
+// The forward solve is folded into the factorization (right-looking): z_k = L_kk^{-1} rhs_k at step k,
+// then rhs_i -= U_ik (V_ik^T z_k) for i > k; it uses the same final factors as the reference's
+// left-looking solve.  Updates with r_ik = 0 or r_jk = 0 are skipped, as in the reference.
+#include "common.hpp"
+#include "panel.hpp"
+#include "tlr_kernels.hpp"
+
+namespace tlr {
+
+constexpr int NB = 32;   // inner panel width of the diagonal-tile factorization
+
+// status[ST_FAILED] holds the first failing diagonal tile (INT_MAX when none).
+__device__ __forceinline__ bool failed(const int* status) {
+  return __hip_atomic_load(&status[ST_FAILED], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0x7fffffff;
+}
+
+// ------------------------------------------------------------------------------------------ POTRF
+// One work group (512 threads) factors the b x b diagonal tile k in place (lower), writes the inverses
+// of its 32 x 32 diagonal blocks to Dinv and sum(log diag L_kk) (tile rows only) to logdet_part[k].
+__global__ __launch_bounds__(512) void k_potrf(SlotArgs sa, int k) {
+  const int slot = blockIdx.z;
+  const SlotPtrs P = sa.slot(slot);
+  if (failed(P.status)) return;
+  const int b = sa.b;
+  const int bk = (int)min<int64_t>(b, sa.n - (int64_t)k * b);
+  double* A = P.D + (int64_t)k * b * b;
+  double* Dv = P.Dinv + (int64_t)k * sa.npan * NB * NB;
+  __shared__ double Lp[NB][512];
+  __shared__ double Lb[NB][NB + 1];
+  __shared__ double Li[NB][NB + 1];
+  __shared__ int s_fail;
+  const int tid = threadIdx.x, w = wave_id(), l = lane_id();
+  double logacc = 0.0;
+  for (int p0 = 0; p0 < b; p0 += NB) {
+    const int wd = min(NB, b - p0);
+    if (w == 0) {
+      double a[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) a[c] = (l < wd && c <= l) ? A[(p0 + l) + (int64_t)(p0 + c) * b] : 0.0;
+      int fail = 0;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (j < wd) {
+          const double d = __shfl(a[j], j, 64);
+          if (!(d > 0.0)) fail = 1;
+          const double ljj = sqrt(d);
+          if (l == j) a[j] = ljj;
+          else if (l > j) a[j] = a[j] / ljj;
+          const double lj = a[j];
+#pragma unroll
+          for (int c = j + 1; c < NB; ++c) {
+            const double lcj = __shfl(lj, c, 64);
+            if (l >= c) a[c] -= lj * lcj;
+          }
+        }
+      }
+      double lg = 0.0;
+      if (l < wd) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          if (c <= l) {
+            Lb[l][c] = a[c];
+            A[(p0 + l) + (int64_t)(p0 + c) * b] = a[c];
+          } else {
+            Lb[l][c] = 0.0;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c == l && p0 + l < bk) lg = log(a[c]);
+      }
+      lg = wave_sum(lg);
+      if (l == 0) { logacc += lg; s_fail = fail; }
+    }
+    __syncthreads();
+    if (s_fail) {
+      if (tid == 0) atomicMin(&P.status[ST_FAILED], k);
+      return;
+    }
+    // inverse of the diagonal block: lane m of wave 0 solves for column m
+    if (w == 0 && l < wd) {
+      double x[NB];
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        double v = 0.0;
+        if (r < wd && r >= l) {
+          double s = (r == l) ? 1.0 : 0.0;
+#pragma unroll
+          for (int c = 0; c < r; ++c) s -= Lb[r][c] * x[c];
+          v = s / Lb[r][r];
+        }
+        x[r] = v;
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        Li[r][l] = x[r];
+        Dv[(p0 / NB) * NB * NB + r + l * NB] = (r < wd) ? x[r] : 0.0;
+      }
+    }
+    __syncthreads();
+    // panel below: L_i,P = A_i,P Linv^T
+    const int i = p0 + wd + tid;
+    if (i < b) {
+      double av[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) av[c] = (c < wd) ? A[i + (int64_t)(p0 + c) * b] : 0.0;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m <= c; ++m) s += av[m] * Li[c][m];
+        if (c < wd) {
+          A[i + (int64_t)(p0 + c) * b] = s;
+          Lp[c][i] = s;
+        } else {
+          Lp[c][i] = 0.0;
+        }
+      }
+    }
+    __syncthreads();
+    // trailing update of the lower triangle of rows/cols >= p0 + wd, 8 x 8 blocks
+    const int t0 = p0 + wd;
+    const int m = b - t0;
+    if (m > 0) {
+      const int n8 = (m + 7) / 8;
+      const int nblk8 = n8 * (n8 + 1) / 2;
+      for (int e = tid; e < nblk8; e += blockDim.x) {
+        int I = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+        while ((I + 1) * (I + 2) / 2 <= e) ++I;
+        while (I * (I + 1) / 2 > e) --I;
+        const int J = e - I * (I + 1) / 2;
+        const int r0 = t0 + 8 * I, c0 = t0 + 8 * J;
+        double acc[8][8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+          for (int c = 0; c < 8; ++c) acc[a][c] = 0.0;
+        for (int cc = 0; cc < wd; ++cc) {
+          double li[8], lj[8];
+#pragma unroll
+          for (int a = 0; a < 8; ++a) li[a] = (r0 + a < b) ? Lp[cc][r0 + a] : 0.0;
+#pragma unroll
+          for (int c = 0; c < 8; ++c) lj[c] = (c0 + c < b) ? Lp[cc][c0 + c] : 0.0;
+#pragma unroll
+          for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) acc[a][c] = fma(li[a], lj[c], acc[a][c]);
+        }
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+          const int r = r0 + a;
+          if (r < b) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+              const int q = c0 + c;
+              if (q <= r) A[r + (int64_t)q * b] -= acc[a][c];
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) P.logdet_part[k] = logacc;
+}
+
+// ------------------------------------------------------------------------------------------ TRSM
+// Item 0: rhs_k -> z_k = L_kk^{-1} rhs_k.  Item m >= 1: tile i = k + m, V_ik <- L_kk^{-1} V_ik, then
+// W_i = V_ik^T V_ik (r x r) and X_i = U_ik W_i (b x r) for the diagonal update.  Blocked forward
+// substitution over the 32-row panels of L_kk using the stored inverses of its diagonal blocks.
+__global__ __launch_bounds__(256) void k_trsm(SlotArgs sa, int k) {
+  const int slot = blockIdx.z;
+  const SlotPtrs P = sa.slot(slot);
+  if (failed(P.status)) return;
+  const int b = sa.b, rc = sa.rank_cap;
+  const int item = blockIdx.x;
+  const double* L = P.D + (int64_t)k * b * b;
+  const double* Dv = P.Dinv + (int64_t)k * sa.npan * NB * NB;
+  double* X;
+  int r;
+  int64_t tik = -1;
+  if (item == 0) {
+    X = P.rhs + (int64_t)k * b;
+    r = 1;
+  } else {
+    const int i = k + item;
+    tik = tri_index(i, k);
+    r = P.rank[tik];
+    X = P.V + tik * sa.fstride;
+  }
+  if (r == 0) return;
+  const int bk = (int)min<int64_t>(b, sa.n - (int64_t)k * b);
+  const int brows = (item == 0) ? bk : b;   // rhs holds exactly the tile's rows
+  __shared__ double tmp[NB * 64];
+  __shared__ double Dl[NB][NB + 1];
+  const int tid = threadIdx.x;
+  for (int p0 = 0; p0 < brows; p0 += NB) {
+    const int wd = min(NB, brows - p0);
+    for (int e = tid; e < NB * NB; e += 256) Dl[e % NB][e / NB] = Dv[(p0 / NB) * NB * NB + e];
+    for (int e = tid; e < NB * r; e += 256) {
+      const int ll = e % NB, c = e / NB;
+      double s = 0.0;
+      if (ll < wd) {
+        const int row = p0 + ll;
+        s = X[row + (int64_t)c * b];
+        double acc = 0.0;
+        for (int q = 0; q < p0; ++q) acc = fma(L[row + (int64_t)q * b], X[q + (int64_t)c * b], acc);
+        s -= acc;
+      }
+      tmp[ll + c * NB] = s;
+    }
+    __syncthreads();
+    for (int e = tid; e < NB * r; e += 256) {
+      const int ll = e % NB, c = e / NB;
+      if (ll < wd) {
+        double s = 0.0;
+        for (int m = 0; m <= ll; ++m) s = fma(Dl[ll][m], tmp[m + c * NB], s);
+        X[(p0 + ll) + (int64_t)c * b] = s;
+      }
+    }
+    __syncthreads();
+  }
+  if (item == 0) return;
+  // W_i = V^T V and X_i = U W
+  double* Wi = P.Wd + (int64_t)(k + item) * rc * rc;
+  double* Xi = P.Xd + (int64_t)(k + item) * b * rc;
+  const double* U = P.U + tik * sa.fstride;
+  __shared__ double Ws[64 * 64];
+  const int w = wave_id(), l = lane_id();
+  for (int e = w; e < r * r; e += 4) {
+    const int a = e % r, c = e / r;
+    double s = 0.0;
+    for (int p = l; p < b; p += 64) s = fma(X[p + (int64_t)a * b], X[p + (int64_t)c * b], s);
+    s = wave_sum(s);
+    if (l == 0) { Ws[a + c * r] = s; Wi[a + c * r] = s; }
+  }
+  __syncthreads();
+  for (int e = tid; e < b * r; e += 256) {
+    const int p = e % b, c = e / b;
+    double s = 0.0;
+    for (int a = 0; a < r; ++a) s = fma(U[p + (int64_t)a * b], Ws[a + c * r], s);
+    Xi[p + (int64_t)c * b] = s;
+  }
+}
+
+// ------------------------------------------------------------------------------ diagonal update
+// A_ii -= X_i U_ik^T on the lower 64 x 64 blocks of tile i = k + 1 + blockIdx.y; block 0 also applies
+// the forward-solve update rhs_i -= U_ik (V_ik^T z_k).
+__global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
+  const int slot = blockIdx.z;
+  const SlotPtrs P = sa.slot(slot);
+  if (failed(P.status)) return;
+  const int b = sa.b;
+  const int i = k + 1 + blockIdx.y;
+  const int64_t tik = tri_index(i, k);
+  const int r = P.rank[tik];
+  if (r == 0) return;
+  const double* U = P.U + tik * sa.fstride;
+  const double* Xi = P.Xd + (int64_t)i * b * sa.rank_cap;
+  double* A = P.D + (int64_t)i * b * b;
+  const int tid = threadIdx.x;
+  int e = blockIdx.x;
+  int I = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= e) ++I;
+  while (I * (I + 1) / 2 > e) --I;
+  const int J = e - I * (I + 1) / 2;
+  const int pb = 64 * I, qb = 64 * J;
+  __shared__ double Xs[64][64];
+  __shared__ double Us[64][64];
+  __shared__ double zv[64];
+  const int tr = tid / 16, tc = tid % 16;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+  for (int e2 = tid; e2 < 64 * r; e2 += 256) {
+    const int rr = e2 % 64, c = e2 / 64;
+    Xs[c][rr] = (pb + rr < b) ? Xi[pb + rr + (int64_t)c * b] : 0.0;
+    Us[c][rr] = (qb + rr < b) ? U[qb + rr + (int64_t)c * b] : 0.0;
+  }
+  __syncthreads();
+  for (int c = 0; c < r; ++c) {
+    double xv[4], uv[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) xv[a] = Xs[c][tr * 4 + a];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) uv[q] = Us[c][tc * 4 + q];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[a][q] = fma(xv[a], uv[q], acc[a][q]);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int p = pb + tr * 4 + a;
+    if (p >= b) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int qq = qb + tc * 4 + q;
+      if (qq < b && qq <= p) A[p + (int64_t)qq * b] -= acc[a][q];
+    }
+  }
+  if (blockIdx.x != 0) return;
+  // forward-solve update of rhs_i
+  const double* V = P.V + tik * sa.fstride;
+  const double* z = P.rhs + (int64_t)k * b;
+  const int bk = (int)min<int64_t>(b, sa.n - (int64_t)k * b);
+  const int bi = (int)min<int64_t>(b, sa.n - (int64_t)i * b);
+  const int w = wave_id(), l = lane_id();
+  for (int c = w; c < r; c += 4) {
+    double s = 0.0;
+    for (int p = l; p < bk; p += 64) s = fma(V[p + (int64_t)c * b], z[p], s);
+    s = wave_sum(s);
+    if (l == 0) zv[c] = s;
+  }
+  __syncthreads();
+  double* rhs = P.rhs + (int64_t)i * b;
+  for (int p = tid; p < bi; p += 256) {
+    double s = 0.0;
+    for (int c = 0; c < r; ++c) s = fma(U[p + (int64_t)c * b], zv[c], s);
+    rhs[p] -= s;
+  }
+}
+
+// ------------------------------------------------------------------------------ recompression
+// One work group per pair (i, j), k < j < i, whose width w = r_ij + r_jk lies in [wlo, whi].
+// Householder QR of both concatenations in registers, one-sided Jacobi SVD of the core R_u R_v^T,
+// truncation by the rule, U_ij = Q_u G[:, perm[:r]], V_ij = Q_v J[:, perm[:r]].
+template <int RPL, int NW, bool BIG>
+__global__ __launch_bounds__(64 * NW) void k_recompress(SlotArgs sa, int k, int wlo, int whi, int rule_fixed,
+                                                       double thr_rel, double thr_abs) {
+  constexpr int WMAX = 8 * NW;
+  const int slot = blockIdx.z;
+  const SlotPtrs P = sa.slot(slot);
+  if (failed(P.status)) return;
+  const int b = sa.b;
+  // pairs (a, b'), 0 <= b' < a, enumerated as tri_index(a, b'); index 0 is (1, 0)
+  int a_, b_;
+  tri_decode((int64_t)blockIdx.x, &a_, &b_);
+  const int i = a_ + k + 1, j = b_ + k + 1;
+  if (i >= sa.nB) return;
+  const int64_t tij = tri_index(i, j), tik = tri_index(i, k), tjk = tri_index(j, k);
+  const int r1 = P.rank[tij], rik = P.rank[tik], rjk = P.rank[tjk];
+  if (rik == 0 || rjk == 0) return;
+  const int w = r1 + rjk;
+  if (w < wlo || w > whi) return;
+  // the three width classes run as separate launches: a pair handled by an earlier class this step
+  // has already changed r_ij and must not be seen again
+  if (P.done_k[tij] == k) return;
+  extern __shared__ double lds[];
+  double* G = lds;
+  double* Wm = lds;                          // aliases G (and J) until the core is formed
+  double* scr = P.scratch + (int64_t)blockIdx.x * sa.scr_stride;
+  double* Qs = scr;                          // b x WMAX explicit Q (V side, then U side)
+  double* J = BIG ? scr + (int64_t)b * 128 : G + WMAX * WMAX;
+  double* Ru = BIG ? J + 128 * 128 : J + WMAX * WMAX;
+  double* Rv = Ru + (BIG ? 128 * 128 : WMAX * WMAX);
+  __shared__ double vbuf[2 * 64 * RPL];
+  __shared__ double tau[WMAX];
+  __shared__ double sig[WMAX];
+  __shared__ int perm[WMAX];
+  __shared__ int flag, s_rank;
+  const int tid = threadIdx.x, wv = wave_id(), ln = lane_id();
+  double* Uij = P.U + tij * sa.fstride;
+  double* Vij = P.V + tij * sa.fstride;
+  const double* Uik = P.U + tik * sa.fstride;
+  const double* Vik = P.V + tik * sa.fstride;
+  const double* Ujk = P.U + tjk * sa.fstride;
+  const double* Vjk = P.V + tjk * sa.fstride;
+  RegPanel<RPL, NW> pan;
+  // ---- V side: [V_ij, U_jk]
+#pragma unroll
+  for (int jj = 0; jj < 8; ++jj) {
+    const int c = wv + NW * jj;
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int p = ln + 64 * t;
+      double v = 0.0;
+      if (p < b && c < w) v = (c < r1) ? Vij[p + (int64_t)c * b] : Ujk[p + (int64_t)(c - r1) * b];
+      pan.a[jj][t] = v;
+    }
+  }
+  pan.geqr2(b, w, vbuf, tau);
+  pan.store_r(Rv, w, w);
+  pan.org2r(b, w, vbuf, tau);
+  pan.store(Qs, b, b, w);
+  // ---- W = V_ik^T V_jk
+  for (int e = wv; e < rik * rjk; e += NW) {
+    const int a = e % rik, c = e / rik;
+    double s = 0.0;
+    for (int p = ln; p < b; p += 64) s = fma(Vik[p + (int64_t)a * b], Vjk[p + (int64_t)c * b], s);
+    s = wave_sum(s);
+    if (ln == 0) Wm[a + c * rik] = s;
+  }
+  __syncthreads();
+  // ---- U side: [U_ij, -U_ik W]
+#pragma unroll
+  for (int jj = 0; jj < 8; ++jj) {
+    const int c = wv + NW * jj;
+    if (c < r1) {
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int p = ln + 64 * t;
+        pan.a[jj][t] = (p < b) ? Uij[p + (int64_t)c * b] : 0.0;
+      }
+    } else if (c < w) {
+      const int cc = c - r1;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int p = ln + 64 * t;
+        double s = 0.0;
+        if (p < b)
+          for (int a = 0; a < rik; ++a) s = fma(Uik[p + (int64_t)a * b], Wm[a + cc * rik], s);
+        pan.a[jj][t] = -s;
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) pan.a[jj][t] = 0.0;
+    }
+  }
+  pan.geqr2(b, w, vbuf, tau);
+  pan.store_r(Ru, w, w);
+  __syncthreads();
+  // ---- core G = R_u R_v^T
+  for (int e = tid; e < w * w; e += blockDim.x) {
+    const int a = e % w, c = e / w;
+    double s = 0.0;
+    for (int m = max(a, c); m < w; ++m) s = fma(Ru[a + m * w], Rv[c + m * w], s);
+    G[a + c * w] = s;
+  }
+  __syncthreads();
+  jacobi_svd<(NW >= 16 ? 16 : (NW >= 8 ? 8 : 4))>(G, w, J, w, w, sig, perm, &flag, 60);
+  if (tid == 0) {
+    double tot = 0.0;
+    for (int c = 0; c < w; ++c) tot += sig[c] * sig[c];
+    const double thr = rule_fixed ? thr_rel * sqrt(tot) : thr_abs;
+    // smallest r with sqrt(sum_{c >= r} s_c^2) <= thr
+    double acc = 0.0;
+    int r = w;
+    for (int c = w - 1; c >= 0; --c) {
+      const double s = sig[perm[c]];
+      const double nacc = acc + s * s;
+      if (sqrt(nacc) <= thr) { acc = nacc; r = c; } else break;
+    }
+    if (r > sa.rank_cap) { atomicMax(&P.status[ST_RANKCAP], 1); r = sa.rank_cap; }
+    s_rank = r;
+  }
+  __syncthreads();
+  const int rn = s_rank;
+  // ---- V_ij = Q_v J[:, perm[:rn]]
+  for (int e = tid; e < b * rn; e += blockDim.x) {
+    const int p = e % b, c = e / b;
+    const int pc = perm[c];
+    double s = 0.0;
+    for (int m = 0; m < w; ++m) s = fma(Qs[p + (int64_t)m * b], J[m + pc * w], s);
+    Vij[p + (int64_t)c * b] = s;
+  }
+  __syncthreads();
+  // ---- U_ij = Q_u G[:, perm[:rn]]
+  pan.org2r(b, w, vbuf, tau);
+  pan.store(Qs, b, b, w);
+  __syncthreads();
+  for (int e = tid; e < b * rn; e += blockDim.x) {
+    const int p = e % b, c = e / b;
+    const int pc = perm[c];
+    double s = 0.0;
+    for (int m = 0; m < w; ++m) s = fma(Qs[p + (int64_t)m * b], G[m + pc * w], s);
+    Uij[p + (int64_t)c * b] = s;
+  }
+  if (tid == 0) { P.rank[tij] = rn; P.done_k[tij] = k; }
+}
+
+// ------------------------------------------------------------------------------------ final
+// logdet = 2 sum_k logdet_part[k]; quad = z^T z; ll = -(quad + logdet + N log 2 pi) / 2.
+// Fixed-order reductions (one work group per slot).
+__global__ __launch_bounds__(256) void k_final(SlotArgs sa) {
+  const int slot = blockIdx.x;
+  const SlotPtrs P = sa.slot(slot);
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  double s = 0.0;
+  for (int64_t p = tid; p < sa.n; p += 256) s = fma(P.rhs[p], P.rhs[p], s);
+  red[tid] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) red[tid] += red[tid + h];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const double quad = red[0];
+    double ld = 0.0;
+    for (int k = 0; k < sa.nB; ++k) ld += P.logdet_part[k];
+    ld *= 2.0;
+    P.result[0] = -0.5 * (quad + ld + (double)sa.n * 1.8378770664093453);
+    P.result[1] = ld;
+    P.result[2] = quad;
+  }
+}
+
+// ------------------------------------------------------------------------------------ launchers
+hipError_t launch_potrf(const SlotArgs& sa, int k, int nslot, hipStream_t s) {
+  hipLaunchKernelGGL(k_potrf, dim3(1, 1, nslot), dim3(512), 0, s, sa, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_trsm(const SlotArgs& sa, int k, int nslot, hipStream_t s) {
+  hipLaunchKernelGGL(k_trsm, dim3(sa.nB - k, 1, nslot), dim3(256), 0, s, sa, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t s) {
+  const int rows = sa.nB - k - 1;
+  if (rows <= 0) return hipSuccess;
+  const int nb64 = (sa.b + 63) / 64;
+  hipLaunchKernelGGL(k_diag_update, dim3(nb64 * (nb64 + 1) / 2, rows, nslot), dim3(256), 0, s, sa, k);
+  return hipGetLastError();
+}
+
+template <int RPL, int NW, bool BIG>
+static hipError_t launch_recomp_t(const SlotArgs& sa, int k, int npairs, int nslot, int wlo, int whi, int rule_fixed,
+                                  double thr_rel, double thr_abs, hipStream_t s) {
+  constexpr int WMAX = 8 * NW;
+  const size_t lds = BIG ? (size_t)WMAX * WMAX * sizeof(double) : (size_t)4 * WMAX * WMAX * sizeof(double);
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)k_recompress<RPL, NW, BIG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_recompress<RPL, NW, BIG>), dim3(npairs, 1, nslot), dim3(64 * NW), lds, s, sa, k, wlo, whi,
+                     rule_fixed, thr_rel, thr_abs);
+  return hipGetLastError();
+}
+
+hipError_t launch_recompress(const SlotArgs& sa, int k, int nslot, int rule_fixed, double thr_rel, double thr_abs,
+                             hipStream_t s) {
+  const int m = sa.nB - k - 1;            // tiles below the panel
+  const int npairs = m * (m - 1) / 2;     // pairs (i, j), k < j < i
+  if (npairs <= 0) return hipSuccess;
+  hipError_t e;
+  if (sa.b <= 256) {
+    e = launch_recomp_t<4, 4, false>(sa, k, npairs, nslot, 1, 32, rule_fixed, thr_rel, thr_abs, s);
+    if (e != hipSuccess) return e;
+    e = launch_recomp_t<4, 8, false>(sa, k, npairs, nslot, 33, 64, rule_fixed, thr_rel, thr_abs, s);
+    if (e != hipSuccess) return e;
+    e = launch_recomp_t<4, 16, true>(sa, k, npairs, nslot, 65, 128, rule_fixed, thr_rel, thr_abs, s);
+  } else {
+    e = launch_recomp_t<8, 4, false>(sa, k, npairs, nslot, 1, 32, rule_fixed, thr_rel, thr_abs, s);
+    if (e != hipSuccess) return e;
+    e = launch_recomp_t<8, 8, false>(sa, k, npairs, nslot, 33, 64, rule_fixed, thr_rel, thr_abs, s);
+    if (e != hipSuccess) return e;
+    e = launch_recomp_t<8, 16, true>(sa, k, npairs, nslot, 65, 128, rule_fixed, thr_rel, thr_abs, s);
+  }
+  return e;
+}
+
+hipError_t launch_final(const SlotArgs& sa, int nslot, hipStream_t s) {
+  hipLaunchKernelGGL(k_final, dim3(nslot), dim3(256), 0, s, sa);
+  return hipGetLastError();
+}
+
+}  // namespace tlr

@@ -1,0 +1,267 @@
+// Work-group-wide small dense linear algebra kept in registers / LDS:
+//  * RegPanel: an m x n panel (m <= 64*RPL, n <= 8*NW) distributed over NW waves; wave w owns the
+//    columns j = w + NW*jj (jj < 8), lane l owns the rows p = l + 64*t (t < RPL).  Householder QR with
+//    LAPACK dlarfg/dgeqr2 conventions, explicit Q in place (dorg2r), all without spilling the panel.
+//  * jacobi_svd: one-sided (Hestenes) Jacobi on the columns of a small square matrix, round-robin
+//    ordering, rotations accumulated into J.
+#pragma once
+#include "common.hpp"
+
+namespace tlr {
+
+template <int RPL, int NW>
+struct RegPanel {
+  static constexpr int CPW = 8;
+  static constexpr int NT = 64 * NW;
+  double a[CPW][RPL];
+
+  __device__ __forceinline__ static int col_of(int w, int jj) { return w + NW * jj; }
+
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj)
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) a[jj][t] = 0.0;
+  }
+
+  // A: column-major, leading dimension lda; entries outside m x n read as zero.
+  __device__ __forceinline__ void load(const double* A, int64_t lda, int m, int n) {
+    const int w = wave_id(), l = lane_id();
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj) {
+      const int j = col_of(w, jj);
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int p = l + 64 * t;
+        a[jj][t] = (j < n && p < m) ? A[p + (int64_t)j * lda] : 0.0;
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(double* A, int64_t lda, int m, int n) const {
+    const int w = wave_id(), l = lane_id();
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj) {
+      const int j = col_of(w, jj);
+      if (j < n) {
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const int p = l + 64 * t;
+          if (p < m) A[p + (int64_t)j * lda] = a[jj][t];
+        }
+      }
+    }
+  }
+
+  // Copy column slot jc (wave-uniform) into x.
+  // Apply the reflector in vb (rows >= c nonzero) with factor tc to every own column j > c.
+  __device__ __forceinline__ void apply_reflector(int c, int n, const double* vb, double tc) {
+    const int w = wave_id(), l = lane_id();
+    double v[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) v[t] = vb[l + 64 * t];
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj) {
+      const int j = col_of(w, jj);
+      if (j > c && j < n) {
+        double d = 0.0;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) d += v[t] * a[jj][t];
+        d = wave_sum(d) * tc;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) a[jj][t] -= d * v[t];
+      }
+    }
+  }
+
+  // Householder QR (dgeqr2): reflector c stored below the diagonal of column c (v_c = 1 implicit),
+  // R on and above the diagonal, tau[c] in LDS.  vbuf: LDS scratch of 2*64*RPL doubles.
+  // Requires m >= n.  Every thread of the work group must call it.  Column c = wo + NW*jc is owned
+  // by wave wo in register slot jc; the slot loop is unrolled so every register index is static.
+  __device__ __forceinline__ void geqr2(int m, int n, double* vbuf, double* tau) {
+    const int w = wave_id(), l = lane_id();
+#pragma unroll
+    for (int jc = 0; jc < CPW; ++jc) {
+      for (int wo = 0; wo < NW; ++wo) {
+        const int c = wo + NW * jc;
+        if (c >= n) break;
+        double* vb = vbuf + (c & 1) * (64 * RPL);
+        if (w == wo) {
+          double part = 0.0, alpha_l = 0.0;
+#pragma unroll
+          for (int t = 0; t < RPL; ++t) {
+            const int p = l + 64 * t;
+            const double x = a[jc][t];
+            if (p > c && p < m) part += x * x;
+            if (p == c) alpha_l = x;
+          }
+          const double sigma = wave_sum(part);
+          const double alpha = wave_sum(alpha_l);  // exactly one lane holds it
+          double beta, tc, scal;
+          if (sigma == 0.0) {
+            tc = 0.0; beta = alpha; scal = 0.0;
+          } else {
+            const double nrm = sqrt(alpha * alpha + sigma);
+            beta = alpha >= 0.0 ? -nrm : nrm;
+            tc = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+          }
+#pragma unroll
+          for (int t = 0; t < RPL; ++t) {
+            const int p = l + 64 * t;
+            double v = 0.0;
+            if (p == c) v = 1.0;
+            else if (p > c && p < m) v = a[jc][t] * scal;
+            vb[p] = v;
+            if (p == c) a[jc][t] = beta;
+            else if (p > c) a[jc][t] = (p < m) ? v : 0.0;
+          }
+          if (l == 0) tau[c] = tc;
+        }
+        __syncthreads();
+        apply_reflector(c, n, vb, tau[c]);
+      }
+    }
+    __syncthreads();
+  }
+
+  // Write the n x n upper triangle R (zeros below) to R (column-major, ldr).  Call after geqr2.
+  __device__ __forceinline__ void store_r(double* R, int ldr, int n) const {
+    const int w = wave_id(), l = lane_id();
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj) {
+      const int j = col_of(w, jj);
+      if (j < n) {
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const int p = l + 64 * t;
+          if (p < n) R[p + j * ldr] = (p <= j) ? a[jj][t] : 0.0;
+        }
+      }
+    }
+  }
+
+  // Same, transposed: Rt[j + p*ldr] = R[p][j].
+  __device__ __forceinline__ void store_rt(double* Rt, int ldr, int n) const {
+    const int w = wave_id(), l = lane_id();
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj) {
+      const int j = col_of(w, jj);
+      if (j < n) {
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const int p = l + 64 * t;
+          if (p < n) Rt[j + p * ldr] = (p <= j) ? a[jj][t] : 0.0;
+        }
+      }
+    }
+  }
+
+  // Explicit Q (m x n) in place from the reflectors left by geqr2 (dorg2r), columns in reverse order.
+  __device__ __forceinline__ void org2r(int m, int n, double* vbuf, const double* tau) {
+    const int w = wave_id(), l = lane_id();
+#pragma unroll
+    for (int jc = CPW - 1; jc >= 0; --jc) {
+      for (int wo = NW - 1; wo >= 0; --wo) {
+        const int c = wo + NW * jc;
+        if (c >= n) continue;
+        double* vb = vbuf + (c & 1) * (64 * RPL);
+        const double tc = tau[c];
+        if (w == wo) {
+#pragma unroll
+          for (int t = 0; t < RPL; ++t) {
+            const int p = l + 64 * t;
+            double v = 0.0;
+            if (p == c) v = 1.0;
+            else if (p > c && p < m) v = a[jc][t];
+            vb[p] = v;
+            // column c of Q = H_c e_c = e_c - tau v
+            a[jc][t] = (p == c) ? 1.0 - tc : ((p > c && p < m) ? -tc * v : 0.0);
+          }
+        }
+        __syncthreads();
+        apply_reflector(c, n, vb, tc);
+      }
+    }
+    __syncthreads();
+  }
+};
+
+// One-sided Jacobi SVD of the n x n matrix G (column-major, ldg; n <= 2*(NT/TP)), rotations
+// accumulated into J (initialised to I here).  On return the columns of G are mutually orthogonal,
+// G_out = G_in J, sig[j] = ||G_out[:, j]|| and perm[0..n) lists columns by descending sig (ties by
+// index).  TP threads cooperate on one column pair.  Returns the number of sweeps (-1: no convergence
+// within max_sweeps).  G, J, sig, perm may live in LDS or global memory; flag is an LDS int.
+template <int TP>
+__device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double* sig, int* perm, int* flag,
+                          int max_sweeps) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int e = tid; e < n * n; e += nt) {
+    const int r = e % n, c = e / n;
+    J[r + c * ldj] = (r == c) ? 1.0 : 0.0;
+  }
+  const int np = (n + 1) & ~1;      // players, padded to even
+  const int npairs = np / 2;
+  const int pi = tid / TP, tl = tid % TP;
+  const double tol = 2.220446049250313e-16 * sqrt((double)n);
+  int sweeps = -1;
+  __syncthreads();
+  for (int s = 0; s < max_sweeps && n > 1; ++s) {
+    if (tid == 0) *flag = 0;
+    __syncthreads();
+    for (int r = 0; r < np - 1; ++r) {
+      if (pi < npairs) {
+        int p, q;
+        if (pi == 0) { p = np - 1; q = r; }
+        else { p = (r + pi) % (np - 1); q = (r - pi + (np - 1)) % (np - 1); }
+        if (p > q) { const int x = p; p = q; q = x; }
+        if (q < n) {
+          double al = 0.0, be = 0.0, ga = 0.0;
+          for (int i = tl; i < n; i += TP) {
+            const double gp = G[i + p * ldg], gq = G[i + q * ldg];
+            al += gp * gp; be += gq * gq; ga += gp * gq;
+          }
+          al = group_sum<TP>(al); be = group_sum<TP>(be); ga = group_sum<TP>(ga);
+          if (ga != 0.0 && al > 0.0 && be > 0.0 && fabs(ga) > tol * sqrt(al) * sqrt(be)) {
+            const double zeta = (be - al) / (2.0 * ga);
+            const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+            const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+            for (int i = tl; i < n; i += TP) {
+              const double gp = G[i + p * ldg], gq = G[i + q * ldg];
+              G[i + p * ldg] = cs * gp - sn * gq;
+              G[i + q * ldg] = sn * gp + cs * gq;
+              const double jp = J[i + p * ldj], jq = J[i + q * ldj];
+              J[i + p * ldj] = cs * jp - sn * jq;
+              J[i + q * ldj] = sn * jp + cs * jq;
+            }
+            if (tl == 0) *flag = 1;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    const int rotated = *flag;
+    __syncthreads();
+    if (!rotated) { sweeps = s + 1; break; }
+  }
+  if (n <= 1) sweeps = 0;
+  for (int j = tid; j < n; j += nt) {
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) { const double g = G[i + j * ldg]; acc += g * g; }
+    sig[j] = sqrt(acc);
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += nt) {
+    const double sj = sig[j];
+    int rank = 0;
+    for (int i = 0; i < n; ++i) {
+      const double si = sig[i];
+      rank += (si > sj) || (si == sj && i < j);
+    }
+    perm[rank] = j;
+  }
+  __syncthreads();
+  return sweeps;
+}
+
+}  // namespace tlr

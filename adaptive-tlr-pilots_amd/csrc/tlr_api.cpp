@@ -1,0 +1,344 @@
+// C ABI of the TLR likelihood: context management and the per-evaluation launch schedule.
+//
+// Schedule of one batch of m theta (one slot per theta), all on the context's stream:
+//   per slot: diagonal tiles; initial compression of every off-diagonal tile (sketch width 64, then 96
+//             for the tiles whose rank left too little oversampling)
+//   for k = 0 .. nB-1 (all slots in each launch): POTRF(k) -> TRSM + forward solve(k) ->
+//             diagonal updates(k) -> recompressions(k) (three width classes)
+//   final reductions; one synchronisation and a small copy back.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/tlrgp.h"
+#include "tlr_kernels.hpp"
+
+using namespace tlr;
+
+static thread_local std::string g_err;
+
+static int set_err(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+  do {                                                                                            \
+    hipError_t _e = (expr);                                                                       \
+    if (_e != hipSuccess)                                                                         \
+      return set_err(TLR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));             \
+  } while (0)
+
+namespace {
+
+constexpr int K1 = 64, K2 = 96;     // sketch widths of the two compression rounds
+constexpr int PMIN = 24;            // spare sketch columns required to accept a rank
+constexpr int NPRODUCT = 4;         // products with T, T^T, T, T^T before the rank decision
+constexpr int RANK_CAP = 64;        // column capacity of the U, V slots
+constexpr int WCAP = 128;           // widest recompression concatenation
+
+struct Slot {
+  SlotPtrs p;
+};
+
+}  // namespace
+
+struct tlr_ctx {
+  int64_t n = 0;
+  int b = 0, nB = 0, npan = 0, max_batch = 0;
+  int64_t ntile = 0, npairs_max = 0;
+  int kcap = 0;
+  hipStream_t stream = nullptr;
+  double *sx = nullptr, *sy = nullptr, *y = nullptr, *omega = nullptr;
+  double *bufA = nullptr, *bufB = nullptr, *bufR = nullptr, *part = nullptr;
+  int* need = nullptr;
+  std::vector<void*> allocs;
+  std::vector<Slot> slots;
+  SlotPtrs* d_slots = nullptr;
+  int64_t scr_stride = 0;
+  ~tlr_ctx() {
+    for (void* p : allocs) hipFree(p);
+    if (stream) hipStreamDestroy(stream);
+  }
+  template <class T>
+  hipError_t alloc(T** p, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T) > 0 ? count * sizeof(T) : 16);
+    if (e == hipSuccess) allocs.push_back(q);
+    *p = (T*)q;
+    return e;
+  }
+  SlotArgs args() const {
+    SlotArgs a;
+    a.n = n; a.b = b; a.nB = nB; a.npan = npan; a.rank_cap = RANK_CAP;
+    a.fstride = (int64_t)b * RANK_CAP;
+    a.scr_stride = scr_stride;
+    a.slots = d_slots;
+    return a;
+  }
+};
+
+// Deterministic standard normal test matrix (splitmix64 + Box-Muller), column-major b x k.
+static void gaussian_matrix(std::vector<double>& out, int rows, int cols, uint64_t seed) {
+  out.resize((size_t)rows * cols);
+  uint64_t s = seed;
+  auto next = [&]() {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  };
+  for (size_t e = 0; e < out.size(); e += 2) {
+    const double u1 = next(), u2 = next();
+    const double r = sqrt(-2.0 * log(u1)), a = 6.283185307179586 * u2;
+    out[e] = r * cos(a);
+    if (e + 1 < out.size()) out[e + 1] = r * sin(a);
+  }
+}
+
+extern "C" {
+
+const char* tlr_last_error(void) { return g_err.c_str(); }
+
+int tlr_device_info(char* buf, int32_t len) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(TLR_ERR_NODEVICE, "no HIP device");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, 0));
+  snprintf(buf, (size_t)len, "%s (%s), %d CUs, %.1f GiB", prop.name, prop.gcnArchName, prop.multiProcessorCount,
+           prop.totalGlobalMem / 1073741824.0);
+  return TLR_OK;
+}
+
+int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, int32_t b, int32_t max_batch) {
+  if (!out || !S || !y || n < 2 || b < 2 || max_batch < 1) return set_err(TLR_ERR_ARG, "invalid argument");
+  if (b > 512) return set_err(TLR_ERR_ARG, "tile size b > 512 is not supported");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(TLR_ERR_NODEVICE, "no HIP device");
+  tlr_ctx* c = new tlr_ctx();
+  auto fail = [&](int code) { delete c; return code; };
+  c->n = n; c->b = b;
+  c->nB = (int)((n + b - 1) / b);
+  c->npan = (b + 31) / 32;
+  c->max_batch = max_batch;
+  c->ntile = (int64_t)c->nB * (c->nB - 1) / 2;
+  const int64_t m = c->nB - 1;
+  c->npairs_max = m * (m - 1) / 2;
+  c->kcap = K2 < b ? K2 : b;
+  if (c->ntile > 65535) return fail(set_err(TLR_ERR_ARG, "more than 65535 off-diagonal tiles"));
+  hipError_t e;
+#define ACHK(expr)                                                                           \
+  do {                                                                                       \
+    e = (expr);                                                                              \
+    if (e != hipSuccess) return fail(set_err(TLR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e))); \
+  } while (0)
+  ACHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  // sites as structure of arrays
+  std::vector<double> hx(n), hy(n);
+  for (int64_t i = 0; i < n; ++i) { hx[i] = S[2 * i]; hy[i] = S[2 * i + 1]; }
+  ACHK(c->alloc(&c->sx, n));
+  ACHK(c->alloc(&c->sy, n));
+  ACHK(c->alloc(&c->y, n));
+  ACHK(hipMemcpy(c->sx, hx.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  ACHK(hipMemcpy(c->sy, hy.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  ACHK(hipMemcpy(c->y, y, n * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> om;
+  gaussian_matrix(om, b, c->kcap, 0x5EED0F7A11ull);
+  ACHK(c->alloc(&c->omega, om.size()));
+  ACHK(hipMemcpy(c->omega, om.data(), om.size() * sizeof(double), hipMemcpyHostToDevice));
+  const int64_t nt = c->ntile > 0 ? c->ntile : 1;
+  const int nbb = (b + 63) / 64;
+  ACHK(c->alloc(&c->bufA, (size_t)nt * b * c->kcap));
+  ACHK(c->alloc(&c->bufB, (size_t)nt * b * c->kcap));
+  ACHK(c->alloc(&c->bufR, (size_t)nt * c->kcap * c->kcap));
+  ACHK(c->alloc(&c->part, (size_t)nt * nbb * nbb * 2));
+  ACHK(c->alloc(&c->need, (size_t)nt));
+  c->scr_stride = (int64_t)b * WCAP + 3 * WCAP * WCAP;
+  const int64_t npm = c->npairs_max > 0 ? c->npairs_max : 1;
+  c->slots.resize(max_batch);
+  for (int s = 0; s < max_batch; ++s) {
+    SlotPtrs& p = c->slots[s].p;
+    ACHK(c->alloc(&p.D, (size_t)c->nB * b * b));
+    ACHK(c->alloc(&p.Dinv, (size_t)c->nB * c->npan * 32 * 32));
+    ACHK(c->alloc(&p.U, (size_t)nt * b * RANK_CAP));
+    ACHK(c->alloc(&p.V, (size_t)nt * b * RANK_CAP));
+    ACHK(c->alloc(&p.rank, (size_t)nt));
+    ACHK(c->alloc(&p.rank0, (size_t)nt));
+    ACHK(c->alloc(&p.done_k, (size_t)nt));
+    ACHK(c->alloc(&p.Wd, (size_t)c->nB * RANK_CAP * RANK_CAP));
+    ACHK(c->alloc(&p.Xd, (size_t)c->nB * b * RANK_CAP));
+    ACHK(c->alloc(&p.rhs, (size_t)n));
+    ACHK(c->alloc(&p.logdet_part, (size_t)c->nB));
+    ACHK(c->alloc(&p.status, (size_t)ST_COUNT));
+    ACHK(c->alloc(&p.result, (size_t)4));
+    ACHK(c->alloc(&p.scratch, (size_t)npm * c->scr_stride));
+  }
+  ACHK(c->alloc(&c->d_slots, (size_t)max_batch));
+  std::vector<SlotPtrs> hp(max_batch);
+  for (int s = 0; s < max_batch; ++s) hp[s] = c->slots[s].p;
+  ACHK(hipMemcpy(c->d_slots, hp.data(), max_batch * sizeof(SlotPtrs), hipMemcpyHostToDevice));
+#undef ACHK
+  *out = c;
+  return TLR_OK;
+}
+
+int tlr_ctx_destroy(tlr_ctx* ctx) {
+  delete ctx;
+  return TLR_OK;
+}
+
+int tlr_ctx_dims(const tlr_ctx* ctx, int64_t* n, int32_t* b, int32_t* n_tiles) {
+  if (!ctx) return set_err(TLR_ERR_ARG, "null context");
+  if (n) *n = ctx->n;
+  if (b) *b = ctx->b;
+  if (n_tiles) *n_tiles = ctx->nB;
+  return TLR_OK;
+}
+
+int tlr_ctx_set_y(tlr_ctx* ctx, const double* y) {
+  if (!ctx || !y) return set_err(TLR_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpy(ctx->y, y, ctx->n * sizeof(double), hipMemcpyHostToDevice));
+  return TLR_OK;
+}
+
+static int compress_slot(tlr_ctx* c, const SlotPtrs& p, const MaternParams& mp, int rule_fixed, double thr_rel,
+                         double thr_abs) {
+  hipStream_t s = c->stream;
+  const int b = c->b;
+  const int64_t nt = c->ntile;
+  HIPCHK(launch_gen_diag(c->sx, c->sy, c->n, b, c->nB, mp, p.D, s));
+  if (nt == 0) return TLR_OK;
+  HIPCHK(launch_fill_int(c->need, nt, 1, s));
+  const int64_t ms = (int64_t)b * c->kcap, rs = (int64_t)c->kcap * c->kcap;
+  const int kmax = c->kcap;
+  const int widths[2] = {K1 < kmax ? K1 : kmax, kmax};
+  const int rounds = widths[0] == widths[1] ? 1 : 2;
+  for (int rd = 0; rd < rounds; ++rd) {
+    const int k = widths[rd];
+    // alternate products: the buffer holding the latest product and the orthonormal basis of the other side
+    double* cur = c->bufA;   // receives T * X
+    double* oth = c->bufB;   // receives T^T * X
+    HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, 0, c->omega, 0, cur, ms, k, c->need, s));
+    int last_op = 0;
+    for (int pidx = 1; pidx < NPRODUCT; ++pidx) {
+      double* src = (last_op == 0) ? cur : oth;
+      double* dst = (last_op == 0) ? oth : cur;
+      HIPCHK(launch_panel_qr(src, ms, b, k, nullptr, 0, 0, c->need, nt, s));
+      last_op ^= 1;
+      HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, last_op, src, ms, dst, ms, k, c->need, s));
+    }
+    // last product M; the other buffer holds the orthonormal basis Qprev
+    double* M = (last_op == 0) ? cur : oth;
+    double* Qprev = (last_op == 0) ? oth : cur;
+    // T ~ M Qprev^T (last op T) or Qprev M^T (last op T^T); residual before M is orthonormalised
+    const double* Arow = (last_op == 0) ? M : Qprev;
+    const double* Bcol = (last_op == 0) ? Qprev : M;
+    HIPCHK(launch_residual(c->sx, c->sy, c->n, b, nt, mp, Arow, Bcol, ms, k, c->part, c->need, s));
+    HIPCHK(launch_panel_qr(M, ms, b, k, c->bufR, rs, last_op == 1 ? 1 : 0, c->need, nt, s));
+    const double* Qrow = (last_op == 0) ? M : Qprev;
+    const double* Qcol = (last_op == 0) ? Qprev : M;
+    HIPCHK(launch_rank_decide(c->bufR, rs, Qrow, Qcol, ms, c->part, b, k, kmax, PMIN, rule_fixed, thr_rel, thr_abs,
+                              p.U, p.V, (int64_t)b * RANK_CAP, RANK_CAP, p.rank, c->need, nullptr, p.status, nt, s));
+  }
+  HIPCHK(launch_copy_int(p.rank0, p.rank, nt, s));
+  return TLR_OK;
+}
+
+int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
+                     double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma, int32_t* ranks_factor) {
+  if (!c || !thetas || !out || m < 1 || m > c->max_batch) return set_err(TLR_ERR_ARG, "invalid argument");
+  if (rule != TLR_RULE_FIXED && rule != TLR_RULE_ADAPTIVE) return set_err(TLR_ERR_ARG, "unknown rule");
+  if (precision != TLR_PREC_FP64) return set_err(TLR_ERR_ARG, "unsupported precision");
+  for (int s = 0; s < m; ++s) {
+    const double* th = thetas + 3 * s;
+    if (!(th[0] > 0.0) || !(th[1] > 0.0) || !(th[2] >= 0.0)) return set_err(TLR_ERR_ARG, "theta must be positive");
+  }
+  if (!(value > 0.0)) return set_err(TLR_ERR_ARG, "rule value must be positive");
+  hipStream_t st = c->stream;
+  const int rule_fixed = rule == TLR_RULE_FIXED;
+  const double bmax = (double)(c->n < c->b ? c->n : c->b);
+  for (int s = 0; s < m; ++s) {
+    const double* th = thetas + 3 * s;
+    const SlotPtrs& p = c->slots[s].p;
+    MaternParams mp;
+    mp.sigma2 = th[0];
+    mp.inv_range_sqrt3 = sqrt(3.0) / th[1];
+    mp.nugget = th[0] * th[2];
+    const double thr_abs = rule_fixed ? 0.0 : value * bmax * th[0] * th[2] / (double)c->n;
+    HIPCHK(launch_fill_int(p.status, ST_COUNT, 0, st));
+    HIPCHK(launch_fill_int(p.status + ST_FAILED, 1, 0x7fffffff, st));
+    HIPCHK(launch_fill_int(p.done_k, c->ntile, -1, st));
+    HIPCHK(hipMemcpyAsync(p.rhs, c->y, c->n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    int rc = compress_slot(c, p, mp, rule_fixed, value, thr_abs);
+    if (rc != TLR_OK) return rc;
+  }
+  const SlotArgs sa = c->args();
+  // recompression thresholds: all slots of one batch share the rule, but thr_abs depends on theta;
+  // launches take one threshold, so batches with the adaptive rule run the recompression per slot
+  for (int k = 0; k < c->nB; ++k) {
+    HIPCHK(launch_potrf(sa, k, m, st));
+    HIPCHK(launch_trsm(sa, k, m, st));
+    HIPCHK(launch_diag_update(sa, k, m, st));
+    if (rule_fixed) {
+      HIPCHK(launch_recompress(sa, k, m, 1, value, 0.0, st));
+    } else {
+      for (int s = 0; s < m; ++s) {
+        const double* th = thetas + 3 * s;
+        const double thr_abs = value * bmax * th[0] * th[2] / (double)c->n;
+        SlotArgs one = sa;
+        one.slots = c->d_slots + s;
+        HIPCHK(launch_recompress(one, k, 1, 0, 0.0, thr_abs, st));
+      }
+    }
+  }
+  HIPCHK(launch_final(sa, m, st));
+  HIPCHK(hipStreamSynchronize(st));
+  int ret = TLR_OK;
+  std::vector<int> rk;
+  for (int s = 0; s < m; ++s) {
+    const SlotPtrs& p = c->slots[s].p;
+    int stv[ST_COUNT];
+    double res[4];
+    HIPCHK(hipMemcpy(stv, p.status, sizeof(stv), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(res, p.result, sizeof(res), hipMemcpyDeviceToHost));
+    if (stv[ST_RANKCAP]) return set_err(TLR_ERR_RANKCAP, "tile rank exceeded the capacity of 64 columns");
+    const int fa = stv[ST_FAILED];
+    const bool ok = fa == 0x7fffffff;
+    out[3 * s + 0] = ok ? res[0] : NAN;
+    out[3 * s + 1] = ok ? res[1] : NAN;
+    out[3 * s + 2] = ok ? res[2] : NAN;
+    if (status) status[s] = ok ? TLR_OK : TLR_NOT_PD;
+    if (failed_at) failed_at[s] = ok ? -1 : fa;
+    if (!ok && m == 1) ret = TLR_NOT_PD;
+    const int nB = c->nB;
+    for (int which = 0; which < 2; ++which) {
+      int32_t* dst = which == 0 ? ranks_sigma : ranks_factor;
+      if (!dst) continue;
+      rk.resize(c->ntile > 0 ? c->ntile : 1);
+      if (c->ntile > 0)
+        HIPCHK(hipMemcpy(rk.data(), which == 0 ? p.rank0 : p.rank, c->ntile * sizeof(int), hipMemcpyDeviceToHost));
+      int32_t* d = dst + (int64_t)s * nB * nB;
+      memset(d, 0, sizeof(int32_t) * nB * nB);
+      for (int i = 1; i < nB; ++i)
+        for (int j = 0; j < i; ++j) d[i * nB + j] = rk[tri_index(i, j)];
+    }
+  }
+  return ret;
+}
+
+int tlr_loglik(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, int32_t precision, double out3[3],
+               int32_t* ranks_sigma, int32_t* ranks_factor, int32_t* failed_at) {
+  int32_t st = 0, fa = -1;
+  const int rc = tlr_loglik_batch(ctx, 1, theta, rule, value, precision, out3, &st, &fa, ranks_sigma, ranks_factor);
+  if (failed_at) *failed_at = fa;
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+// Kernel launchers and the device-side layout of one likelihood evaluation ("slot").
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "common.hpp"
+
+namespace tlr {
+
+enum { ST_RANKCAP = 0, ST_FAILED = 1, ST_COUNT = 4 };
+
+// Device buffers of one slot (one theta being evaluated).  Layouts (all FP64 column-major):
+//   D      nB tiles of b x b: diagonal tiles, factored in place into L_kk (lower)
+//   Dinv   nB x npan blocks of 32 x 32: inverses of the diagonal blocks of L_kk
+//   U, V   ntile slots of b x rank_cap: off-diagonal factors, tile (i,j) at tri_index(i,j)
+//   rank   ntile ints (current rank), rank0 the ranks after the initial compression
+//   Wd, Xd nB x (rank_cap^2) and nB x (b x rank_cap): W_i = V_ik^T V_ik and U_ik W_i of step k
+//   rhs    N: y on entry, z = L^{-1} y on exit
+//   scratch per-pair workspace of the recompression
+struct SlotPtrs {
+  double* D;
+  double* Dinv;
+  double* U;
+  double* V;
+  int* rank;
+  int* rank0;
+  int* done_k;
+  double* Wd;
+  double* Xd;
+  double* rhs;
+  double* logdet_part;
+  int* status;
+  double* result;   // [loglik, logdet, quad]
+  double* scratch;
+};
+
+struct SlotArgs {
+  int64_t n;
+  int b, nB, npan, rank_cap;
+  int64_t fstride;      // b * rank_cap
+  int64_t scr_stride;   // doubles of scratch per recompression pair
+  const SlotPtrs* slots;
+  __device__ __forceinline__ SlotPtrs slot(int s) const { return slots[s]; }
+};
+
+// compress.hip
+hipError_t launch_gen_diag(const double* sx, const double* sy, int64_t n, int b, int nB, MaternParams mp, double* D,
+                           hipStream_t s);
+hipError_t launch_matern_product(const double* sx, const double* sy, int64_t n, int b, int64_t ntile, MaternParams mp,
+                                 int op, const double* X, int64_t xstride, double* out, int64_t ostride, int k,
+                                 const int* active, hipStream_t s);
+hipError_t launch_panel_qr(double* M, int64_t mstride, int b, int k, double* R, int64_t rstride, int r_transposed,
+                           const int* active, int64_t ntile, hipStream_t s);
+hipError_t launch_residual(const double* sx, const double* sy, int64_t n, int b, int64_t ntile, MaternParams mp,
+                           const double* A, const double* B, int64_t stride, int k, double* part, const int* active,
+                           hipStream_t s);
+size_t rank_decide_lds(int k);
+hipError_t launch_rank_decide(const double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
+                              int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
+                              double thr_rel, double thr_abs, double* U, double* V, int64_t fstride, int rank_cap,
+                              int* rank, int* need, double* tail_out, int* status, int64_t ntile, hipStream_t s);
+hipError_t launch_fill_int(int* p, int64_t count, int value, hipStream_t s);
+hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t s);
+
+// factor.hip
+hipError_t launch_potrf(const SlotArgs& sa, int k, int nslot, hipStream_t s);
+hipError_t launch_trsm(const SlotArgs& sa, int k, int nslot, hipStream_t s);
+hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t s);
+hipError_t launch_recompress(const SlotArgs& sa, int k, int nslot, int rule_fixed, double thr_rel, double thr_abs,
+                             hipStream_t s);
+hipError_t launch_final(const SlotArgs& sa, int nslot, hipStream_t s);
+
+}  // namespace tlr

@@ -1,0 +1,205 @@
+"""Host-side mirror of the reference's likelihood interface, running on the HIP library.
+
+Reference call sites reproduced here (signatures, argument meaning and failure conventions):
+  tlr_loglik(S, y, theta, bounds, rule, value) -> (ll | None, fac | None, info)
+      codes/17_tlr_cholesky_pilot.py:259-266 (factorization 17:137-215)
+  neg_loglik_tlr(logth, S, y, bounds, rule, value, trace) -> -ll | 1e12
+      codes/17_tlr_cholesky_pilot.py:279-290
+  Lik(S, y, bounds, rule, value)(theta) -> ll | -inf, with .last
+      codes/38_tlr_cholesky_mcmc.py:39-56
+  tile_bounds(n, b)                     codes/11_tlr_rank_rules.py:69-71
+  morton_order(xy, order=10)            codes/11_tlr_rank_rules.py:40-53
+
+Differences: fac is a TLRFactor summary (ranks, logdet, quad) -- the factor itself stays in device
+memory; info carries the reference's keys plus ranks_sigma / ranks_factor (nB x nB), logdet and quad.
+"""
+import hashlib
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, f64, iptr, lib
+
+RULES = {"fixed": _lib.TLR_RULE_FIXED, "adaptive": _lib.TLR_RULE_ADAPTIVE}
+PRECISIONS = {"fp64": _lib.TLR_PREC_FP64}
+
+
+def tile_bounds(n, b):
+    """[(0,b), (b,2b), ..., (., n)], the last tile possibly short (11:69-71)."""
+    edges = list(range(0, n, b)) + [n]
+    return [(edges[t], edges[t + 1]) for t in range(len(edges) - 1)]
+
+
+def _spread(v):
+    v = v & np.uint64(0xFFFFFFFF)
+    for sh, mask in ((16, 0x0000FFFF0000FFFF), (8, 0x00FF00FF00FF00FF), (4, 0x0F0F0F0F0F0F0F0F),
+                     (2, 0x3333333333333333), (1, 0x5555555555555555)):
+        v = (v | (v << np.uint64(sh))) & np.uint64(mask)
+    return v
+
+
+def morton_order(xy, order=10):
+    """Stable argsort of the Z-order key on a 2^order grid, x bits even, y bits odd (11:40-53)."""
+    n = 1 << order
+    gx = np.clip((xy[:, 0] * n).astype(np.int64), 0, n - 1).astype(np.uint64)
+    gy = np.clip((xy[:, 1] * n).astype(np.int64), 0, n - 1).astype(np.uint64)
+    return np.argsort(_spread(gx) | (_spread(gy) << np.uint64(1)), kind="stable")
+
+
+def _tile_size_from_bounds(n, bounds):
+    """The uniform tile size b of bounds == tile_bounds(n, b); anything else is rejected."""
+    if not bounds:
+        raise ValueError("empty bounds")
+    b = bounds[0][1] - bounds[0][0]
+    if list(map(tuple, bounds)) != tile_bounds(n, b):
+        raise ValueError("bounds must equal tile_bounds(N, b) (uniform tiles, last one possibly short)")
+    return b
+
+
+class TLRFactor:
+    """Summary of a completed device-resident TLR factor: per-tile ranks, log-determinant, quadratic form."""
+
+    def __init__(self, ranks, logdet, quad):
+        self.ranks = ranks
+        self.logdet = logdet
+        self.quad = quad
+
+    def __repr__(self):
+        nb = self.ranks.shape[0]
+        return f"TLRFactor(nB={nb}, mean_rank={self.ranks[np.tril_indices(nb, -1)].mean():.3f})"
+
+
+def rank_info(ranks_sigma, ranks_factor, n, b, completed):
+    """The reference's info keys from rank arrays: stored_sigma / max_rank_sigma (17:159-167),
+    stored_factor / mean_rank_factor / max_rank_factor (17:206-209)."""
+    bounds = tile_bounds(n, b)
+    nb = len(bounds)
+    sizes = np.array([e - s for s, e in bounds])
+    il = np.tril_indices(nb, -1)
+    diag = int((sizes ** 2).sum())
+    rs = ranks_sigma[il]
+    maxdim = np.maximum(sizes[il[0]], sizes[il[1]])
+    info = dict(stored_sigma=int(diag + (2 * rs * maxdim).sum()), max_rank_sigma=int(rs.max()) if rs.size else 0,
+                ranks_sigma=ranks_sigma)
+    if completed:
+        rf = ranks_factor[il]
+        info.update(stored_factor=int(diag + (2 * rf * sizes[il[0]]).sum()),
+                    mean_rank_factor=float(rf.mean()) if rf.size else 0.0,
+                    max_rank_factor=int(rf.max()) if rf.size else 0, ranks_factor=ranks_factor)
+    return info
+
+
+class TLRContext:
+    """Device-resident sites and observations plus workspace for up to max_batch simultaneous theta."""
+
+    def __init__(self, S, y, b, max_batch=1):
+        S = f64(S)
+        y = f64(y)
+        if S.ndim != 2 or S.shape[1] != 2 or y.shape != (S.shape[0],):
+            raise ValueError("S must be (N, 2) and y (N,)")
+        self.n = S.shape[0]
+        self.b = int(b)
+        self.max_batch = int(max_batch)
+        self._ptr = _lib.ctypes.c_void_p()
+        check(lib.tlr_ctx_create(_lib.ctypes.byref(self._ptr), dptr(S), dptr(y), self.n, self.b, self.max_batch))
+        nb = _lib.ctypes.c_int32()
+        check(lib.tlr_ctx_dims(self._ptr, None, None, _lib.ctypes.byref(nb)))
+        self.nB = nb.value
+
+    def close(self):
+        if self._ptr:
+            lib.tlr_ctx_destroy(self._ptr)
+            self._ptr = _lib.ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_y(self, y):
+        check(lib.tlr_ctx_set_y(self._ptr, dptr(f64(y, (self.n,)))))
+
+    def loglik_batch(self, thetas, rule, value, precision="fp64", ranks=True):
+        """Evaluate m <= max_batch theta in one pass.  Returns dict of arrays: loglik, logdet, quad (NaN when
+        not PD), completed (bool), failed_at (-1 when completed), ranks_sigma / ranks_factor (m, nB, nB)."""
+        th = f64(np.atleast_2d(thetas))
+        m = th.shape[0]
+        if th.shape[1] != 3 or m > self.max_batch:
+            raise ValueError(f"thetas must be (m, 3) with m <= {self.max_batch}")
+        out = np.empty((m, 3))
+        st = np.empty(m, np.int32)
+        fa = np.empty(m, np.int32)
+        rs = np.zeros((m, self.nB, self.nB), np.int32) if ranks else None
+        rf = np.zeros((m, self.nB, self.nB), np.int32) if ranks else None
+        check(lib.tlr_loglik_batch(self._ptr, m, dptr(th), RULES[rule], float(value), PRECISIONS[precision],
+                                   dptr(out), iptr(st), iptr(fa), iptr(rs), iptr(rf)))
+        return dict(loglik=out[:, 0], logdet=out[:, 1], quad=out[:, 2], completed=st == _lib.TLR_OK,
+                    failed_at=fa, ranks_sigma=rs, ranks_factor=rf)
+
+    def loglik(self, theta, rule, value, precision="fp64"):
+        """One theta: (ll | None, TLRFactor | None, info) as tlr_loglik of 17:259-266."""
+        r = self.loglik_batch(np.asarray(theta, float)[None, :], rule, value, precision)
+        completed = bool(r["completed"][0])
+        info = rank_info(r["ranks_sigma"][0], r["ranks_factor"][0], self.n, self.b, completed)
+        info["completed"] = completed
+        if not completed:
+            info["failed_at"] = int(r["failed_at"][0])
+            return None, None, info
+        info.update(logdet=float(r["logdet"][0]), quad=float(r["quad"][0]))
+        fac = TLRFactor(r["ranks_factor"][0], info["logdet"], info["quad"])
+        return float(r["loglik"][0]), fac, info
+
+
+_CTX_CACHE = {}
+
+
+def _context_for(S, y, b, max_batch=1):
+    """Contexts are cached by the content of (S, y) and b: a sampler calling the likelihood repeatedly
+    on one field uploads it once."""
+    S = f64(S)
+    y = f64(y)
+    h = hashlib.blake2b(S.tobytes(), digest_size=16)
+    h.update(y.tobytes())
+    key = (h.hexdigest(), int(b), S.shape[0])
+    ctx = _CTX_CACHE.get(key)
+    if ctx is None or ctx.max_batch < max_batch:
+        if len(_CTX_CACHE) >= 4:
+            _CTX_CACHE.pop(next(iter(_CTX_CACHE))).close()
+        ctx = TLRContext(S, y, b, max_batch)
+        _CTX_CACHE[key] = ctx
+    return ctx
+
+
+def tlr_loglik(S, y, theta, bounds, rule, value, precision="fp64"):
+    """TLR log-likelihood on the GPU: (ll | None, fac | None, info).  Drop-in for 17:259-266."""
+    S = f64(S)
+    b = _tile_size_from_bounds(S.shape[0], bounds)
+    return _context_for(S, y, b).loglik(theta, rule, value, precision)
+
+
+def neg_loglik_tlr(logth, S, y, bounds, rule, value, trace):
+    """Objective of the MLE: -ll, or 1e12 when the factorization fails (17:279-290; dense rule excluded)."""
+    theta = np.exp(logth)
+    trace["evals"] += 1
+    ll, _, _ = tlr_loglik(S, y, theta, bounds, rule, value)
+    if ll is None:
+        trace["failures"] += 1
+        return 1e12
+    return -ll
+
+
+class Lik:
+    """Sampler-facing wrapper (38:39-56): lik(theta) -> ll, or -inf when the factorization fails."""
+
+    def __init__(self, S, y, bounds, rule, value, precision="fp64"):
+        self.S, self.y, self.bounds, self.rule, self.value = f64(S), f64(y), bounds, rule, value
+        self.precision = precision
+        self.last = {}
+
+    def __call__(self, theta):
+        t0 = time.time()
+        ll, fac, info = tlr_loglik(self.S, self.y, theta, self.bounds, self.rule, self.value, self.precision)
+        self.last = dict(ok=ll is not None, time=time.time() - t0, fac=fac, info=info)
+        return -np.inf if ll is None else ll

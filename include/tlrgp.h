@@ -1,0 +1,85 @@
+/*
+ * tlrgp: C ABI of the MI355X adaptive-rank tile low-rank (TLR) Gaussian-process log-likelihood.
+ *
+ * The reference has no FFI: its boundary is the Python call
+ *   tlr_loglik(S, y, theta, bounds, rule, value) -> (ll | None, (L_diag, L_off) | None, info)
+ * in codes/17_tlr_cholesky_pilot.py:259-266 (factorization 17:137-215, forward solve 17:230-240), wrapped
+ * for samplers by Lik.__call__ in codes/38_tlr_cholesky_mcmc.py:43-56 (failure -> -inf) and for
+ * optimizers by neg_loglik_tlr in 17:279-290 (failure -> 1e12).  The entry points below are what a
+ * ctypes binding of that call needs; the Python facade in adaptive-tlr-pilots_amd/tlrgp restores the
+ * reference signatures on top of them.
+ *
+ * Conventions
+ *  - S is float64, shape (N, 2), row-major (x0, y0, x1, y1, ...), Morton-ordered by the caller; y is
+ *    float64 (N).  Both are host pointers and are copied at context creation; the caller keeps ownership.
+ *  - Tiles follow tile_bounds(N, b) of codes/11_tlr_rank_rules.py:69-71: nB = ceil(N / b) tiles, the last
+ *    one possibly short; the rank rule uses the maximum tile size b (17:143).
+ *  - theta = (sigma2, range, eta); the covariance is sigma2 * matern32(h, range) + sigma2 * eta * I.
+ *  - rule TLR_RULE_FIXED: value = eps (relative Frobenius tolerance per tile, 17:114, 17:131);
+ *    rule TLR_RULE_ADAPTIVE: value = tau, absolute budget tau * b * sigma2 * eta / N (17:150).
+ *  - Rank arrays are dense nB x nB int32, row-major, entry [i * nB + j] for the tile (i, j), i > j;
+ *    other entries are 0.  ranks_sigma: after the initial compression; ranks_factor: of the factor L.
+ *  - Return values: TLR_OK when the factorization completed; TLR_NOT_PD when a diagonal tile was not
+ *    positive definite (failed_at receives the first failing tile k, as info["failed_at"] in 17:175);
+ *    a negative value on error, with a message from tlr_last_error().
+ *  - One context per host thread; calls on one context are serialised by the caller.
+ */
+#ifndef TLRGP_H
+#define TLRGP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TLR_OK 0
+#define TLR_NOT_PD 1
+#define TLR_ERR_ARG (-1)
+#define TLR_ERR_HIP (-2)
+#define TLR_ERR_RANKCAP (-3)
+#define TLR_ERR_NODEVICE (-4)
+
+#define TLR_RULE_FIXED 0
+#define TLR_RULE_ADAPTIVE 1
+
+#define TLR_PREC_FP64 0
+
+typedef struct tlr_ctx tlr_ctx;
+
+/* Upload sites and observations; allocate device workspace for up to max_batch simultaneous theta.
+ * Replaces the per-call setup of tlr_cholesky (17:141-150) and the data held by Lik (38:40-42). */
+int tlr_ctx_create(tlr_ctx** ctx, const double* S, const double* y, int64_t n, int32_t b, int32_t max_batch);
+
+int tlr_ctx_destroy(tlr_ctx* ctx);
+
+/* N, b and the number of tiles nB of a context. */
+int tlr_ctx_dims(const tlr_ctx* ctx, int64_t* n, int32_t* b, int32_t* n_tiles);
+
+/* Replace the observations y (N float64, host) of a context. */
+int tlr_ctx_set_y(tlr_ctx* ctx, const double* y);
+
+/* TLR log-likelihood at one theta: replaces tlr_loglik (17:259-266).
+ * out3 receives (loglik, logdet, quad) with loglik = -(quad + logdet + N log 2 pi) / 2, logdet =
+ * 2 sum_k sum log diag L_kk and quad = z^T z, z = L^{-1} y (17:263-265).  ranks_sigma / ranks_factor
+ * (nB * nB each) may be NULL. */
+int tlr_loglik(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, int32_t precision, double out3[3],
+               int32_t* ranks_sigma, int32_t* ranks_factor, int32_t* failed_at);
+
+/* m theta at once (m <= max_batch), each with its own ranks; thetas is m x 3, out is m x 3, status and
+ * failed_at are m-vectors (per-theta TLR_OK / TLR_NOT_PD).  ranks_* are m x nB x nB or NULL.
+ * The batched form of repeated tlr_loglik calls (Lik.__call__ per proposal, 38:43-56). */
+int tlr_loglik_batch(tlr_ctx* ctx, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
+                     double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma, int32_t* ranks_factor);
+
+/* Message of the last error on the calling thread. */
+const char* tlr_last_error(void);
+
+/* Name of device 0 and its compute units, for reports. */
+int tlr_device_info(char* buf, int32_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

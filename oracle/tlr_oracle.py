@@ -1,0 +1,362 @@
+"""CPU oracle for the adaptive-rank TLR Gaussian-process log-likelihood.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: ``tests/``, the smoke entry point and the
+CPU-baseline leg of the benchmark may import it; the product path (the HIP library and its Python
+facade) never does.
+
+It restates, in NumPy/SciPy, the semantics of the reference scripts
+
+* ``codes/11_tlr_rank_rules.py``  -- kernel, distances, Morton order, tile bounds, dense log-likelihood,
+  the data generator recipe;
+* ``codes/17_tlr_cholesky_pilot.py`` -- per-tile truncated SVD, QR+SVD recompression, the right-looking
+  TLR Cholesky, the tile forward/backward solves and the TLR log-likelihood.
+
+The storage-precision ``mixed`` rule of 17 is not part of the hot path and is not restated.  In
+addition to the reference's ``info`` dictionary the factorization reports the per-tile ranks (initial
+and final, as dense lower-triangular nB x nB int arrays), the log-determinant and the quadratic form,
+which the reference computes but never writes.
+
+Pinning (see ``tests/test_oracle_golden.py``): the dense log-likelihood, the TLR log-likelihood, stored
+values, mean/max factor rank, completion flags and failing tile of every rule are checked against the
+reference's committed result JSONs on the reference's own committed inputs; Morton order, tile bounds
+and the kernel against closed forms; the rank rule against matrices with known singular values;
+``recompress`` against a direct SVD of the product; the TLR factor against the dense Cholesky at a
+vanishing tolerance.  The analytic dense gradient is pinned by central finite differences of
+``loglik_dense``.
+"""
+import numpy as np
+from scipy.linalg import cho_factor, cho_solve, cholesky, solve_triangular, LinAlgError
+from scipy.linalg import svd as _scipy_svd
+
+SQRT3 = np.sqrt(3.0)
+LOG2PI = np.log(2.0 * np.pi)
+
+
+# ----------------------------------------------------------------------------- model primitives
+def matern32(h, r):
+    """Matern nu=3/2 correlation (1 + x) exp(-x), x = sqrt(3) h / r.  Follows 11:35-37."""
+    x = SQRT3 * h / r
+    return (1.0 + x) * np.exp(-x)
+
+
+def dist(A, B):
+    """Euclidean distances between rows of A (m,2) and B (n,2), sqrt(max(d2, 0)).  Follows 11:56-58."""
+    dx = A[:, None, 0] - B[None, :, 0]
+    dy = A[:, None, 1] - B[None, :, 1]
+    return np.sqrt(np.maximum(dx * dx + dy * dy, 0.0))
+
+
+def cov_full(S, theta):
+    """Dense covariance sigma2 * matern32 + sigma2 * eta * I.  Follows 11:61-65."""
+    sigma2, rng_, eta = theta
+    K = sigma2 * matern32(dist(S, S), rng_)
+    K[np.diag_indices_from(K)] += sigma2 * eta
+    return K
+
+
+def _spread_bits(v):
+    """Interleave zeros between the low 32 bits of v (uint64), the usual magic-mask bit spread."""
+    v = v & np.uint64(0xFFFFFFFF)
+    for shift, mask in ((16, 0x0000FFFF0000FFFF), (8, 0x00FF00FF00FF00FF), (4, 0x0F0F0F0F0F0F0F0F),
+                        (2, 0x3333333333333333), (1, 0x5555555555555555)):
+        v = (v | (v << np.uint64(shift))) & np.uint64(mask)
+    return v
+
+
+def morton_order(xy, order=10):
+    """Stable argsort of the Z-order key on a 2^order grid; x bits even, y bits odd.  Follows 11:40-53."""
+    n = 1 << order
+    gx = np.clip((xy[:, 0] * n).astype(np.int64), 0, n - 1).astype(np.uint64)
+    gy = np.clip((xy[:, 1] * n).astype(np.int64), 0, n - 1).astype(np.uint64)
+    key = _spread_bits(gx) | (_spread_bits(gy) << np.uint64(1))
+    return np.argsort(key, kind="stable")
+
+
+def tile_bounds(n, b):
+    """[(0,b), (b,2b), ..., (., n)], last tile possibly short.  Follows 11:69-71."""
+    edges = list(range(0, n, b)) + [n]
+    return [(edges[t], edges[t + 1]) for t in range(len(edges) - 1)]
+
+
+def loglik_dense(K, y):
+    """Exact Gaussian log-likelihood through a dense Cholesky; (None, None) if K is not PD.  Follows 11:112-120."""
+    try:
+        c, low = cho_factor(K, lower=True, check_finite=False)
+    except LinAlgError:
+        return None, None
+    alpha = cho_solve((c, low), y, check_finite=False)
+    logdet = 2.0 * np.log(np.diag(c)).sum()
+    return -0.5 * (y @ alpha + logdet + len(y) * LOG2PI), (c, low)
+
+
+def loglik_dense_parts(S, y, theta):
+    """Dense log L with its log-determinant and quadratic form (the split the TLR path also reports)."""
+    K = cov_full(S, theta)
+    try:
+        c, low = cho_factor(K, lower=True, check_finite=False)
+    except LinAlgError:
+        return None
+    z = solve_triangular(c, y, lower=True, check_finite=False)
+    logdet = 2.0 * np.log(np.diag(c)).sum()
+    quad = float(z @ z)
+    return dict(loglik=-0.5 * (quad + logdet + len(y) * LOG2PI), logdet=float(logdet), quad=quad)
+
+
+def loglik_dense_grad(S, y, theta):
+    """Dense log L and its gradient in (log sigma2, log range, log eta).
+
+    dl/dlog(t) = 1/2 a^T K_t a - 1/2 tr(K^{-1} K_t), a = K^{-1} y, with
+    K_{log sigma2} = K, K_{log eta} = sigma2 eta I and K_{log range} = sigma2 x^2 exp(-x)
+    (x = sqrt(3) h / r: d/dlog r of (1+x)e^{-x} is x^2 e^{-x}).  The mathematics of the model; the
+    reference has no gradient."""
+    sigma2, r, eta = theta
+    n = len(y)
+    x = SQRT3 * dist(S, S) / r
+    E = np.exp(-x)
+    K = sigma2 * (1.0 + x) * E
+    K[np.diag_indices_from(K)] += sigma2 * eta
+    c, low = cho_factor(K, lower=True, check_finite=False)
+    a = cho_solve((c, low), y, check_finite=False)
+    Kinv = cho_solve((c, low), np.eye(n), check_finite=False)
+    ll = -0.5 * (y @ a + 2.0 * np.log(np.diag(c)).sum() + n * LOG2PI)
+    Kr = sigma2 * x * x * E
+    g_s = 0.5 * (a @ K @ a) - 0.5 * n
+    g_r = 0.5 * (a @ Kr @ a) - 0.5 * np.sum(Kinv * Kr)
+    g_e = 0.5 * sigma2 * eta * (a @ a) - 0.5 * sigma2 * eta * np.trace(Kinv)
+    return float(ll), np.array([g_s, g_r, g_e])
+
+
+# ----------------------------------------------------------------------------- data generators
+def simulate_reference(n, n_pred=300, theta=(1.0, 0.3, 0.05), seed=20261020):
+    """The reference's generator recipe (11:185-200 = 17:313-325 = 38:28-36): PCG64 stream,
+    S_all ~ U[0,1]^2, y_all = chol(K_all) z, random permutation into observed/held-out, Morton order
+    of the observed sites.  Returns (S, y, Sp, yp)."""
+    rng = np.random.default_rng(seed)
+    theta = np.asarray(theta, float)
+    n_tot = n + n_pred
+    S_all = rng.random((n_tot, 2))
+    L = np.linalg.cholesky(cov_full(S_all, theta))
+    y_all = L @ rng.standard_normal(n_tot)
+    del L
+    perm = rng.permutation(n_tot)
+    obs, hold = perm[:n], perm[n:]
+    S, y = S_all[obs], y_all[obs]
+    order = morton_order(S)
+    return S[order], y[order], S_all[hold], y_all[hold]
+
+
+def simulate_seed0(n, theta=(1.0, 0.3, 0.05), seed=0):
+    """Synthetic field of the benchmark configurations: S ~ U[0,1]^2 from default_rng(seed), Morton
+    ordered (stable sort), y = chol(Sigma(theta)) z drawn in dense FP64 on the ordered sites."""
+    rng = np.random.default_rng(seed)
+    S = rng.random((n, 2))
+    S = S[morton_order(S)]
+    L = np.linalg.cholesky(cov_full(S, np.asarray(theta, float)))
+    y = L @ rng.standard_normal(n)
+    return S, y
+
+
+# ----------------------------------------------------------------------------- rank rules
+def _tail_norms(s):
+    """tail[r] = ||(s_r, s_{r+1}, ...)||_2 for r = 0..len(s), tail[len(s)] = 0 (17:113, 17:130)."""
+    t = np.sqrt(np.cumsum((s * s)[::-1])[::-1])
+    return np.append(t, 0.0)
+
+
+def _rank_of(s, rule, thr_rel, thr_abs):
+    """Smallest r with tail[r] <= thr; thr = eps ||s|| (fixed) or the absolute budget (17:114-115)."""
+    tail = _tail_norms(s)
+    thr = thr_rel * np.sqrt((s * s).sum()) if rule == "fixed" else thr_abs
+    r = int(np.argmax(tail <= thr))
+    return r, float(tail[r])
+
+
+def _svd(A, full):
+    try:
+        return np.linalg.svd(A, full_matrices=full)
+    except np.linalg.LinAlgError:   # gesdd non-convergence: gesvd driver, as 17:111-112 / 17:128-129
+        return _scipy_svd(A, full_matrices=full, lapack_driver="gesvd")
+
+
+def truncate_svd(T, rule, thr_rel, thr_abs):
+    """Rank-r truncation of a dense tile; T ~ U V^T with U = U_r s_r, V = V_r.  Follows 17:107-116."""
+    U, s, Vt = _svd(T, False)
+    r, err = _rank_of(s, rule, thr_rel, thr_abs)
+    return U[:, :r] * s[:r], Vt[:r].T, err
+
+
+def recompress(U, V, rule, thr_rel, thr_abs):
+    """Smallest-rank re-truncation of U V^T via QR(U), QR(V) and the SVD of R_u R_v^T.  Follows 17:119-133."""
+    if U.shape[1] == 0:
+        return U, V, 0.0
+    Qu, Ru = np.linalg.qr(U)
+    Qv, Rv = np.linalg.qr(V)
+    Uw, s, Vwt = _svd(Ru @ Rv.T, True)
+    r, err = _rank_of(s, rule, thr_rel, thr_abs)
+    return Qu @ (Uw[:, :r] * s[:r]), Qv @ Vwt[:r].T, err
+
+
+def rule_thresholds(rule, value, theta, n, b):
+    """(thr_rel, thr_abs) of 17:149-150: fixed -> eps; adaptive -> tau * b * sigma2 * eta / N (b = max tile)."""
+    sigma2, _, eta = theta
+    if rule == "fixed":
+        return value, None
+    if rule == "adaptive":
+        return None, value * b * sigma2 * eta / n
+    raise ValueError(f"unknown rule {rule!r}")
+
+
+# ----------------------------------------------------------------------------- TLR Cholesky
+def tlr_cholesky(S, theta, bounds, rule, value):
+    """Right-looking TLR Cholesky of Sigma(theta) without forming it.  Follows 17:137-215.
+
+    Returns (L_diag, L_off, info); (None, None, info) with completed=False, failed_at=k when the
+    diagonal tile k is not positive definite.  info adds 'ranks_sigma' and 'ranks_factor' (nB x nB
+    int arrays, lower triangle) to the reference's keys."""
+    n = S.shape[0]
+    nB = len(bounds)
+    b = max(e - s for s, e in bounds)
+    sigma2, r, eta = theta
+    thr_rel, thr_abs = rule_thresholds(rule, value, theta, n, b)
+    A_diag, A_off = [], {}
+    ranks0 = np.zeros((nB, nB), np.int64)
+    stored0 = 0
+    # initial tiles, row-major over the lower block triangle (17:155-166)
+    for i, (i0, i1) in enumerate(bounds):
+        D = sigma2 * matern32(dist(S[i0:i1], S[i0:i1]), r)
+        D[np.diag_indices_from(D)] += sigma2 * eta
+        A_diag.append(D)
+        stored0 += (i1 - i0) ** 2
+        for j, (j0, j1) in enumerate(bounds[:i]):
+            T = sigma2 * matern32(dist(S[i0:i1], S[j0:j1]), r)
+            U, V, _ = truncate_svd(T, rule, thr_rel, thr_abs)
+            A_off[(i, j)] = (U, V)
+            ranks0[i, j] = U.shape[1]
+            stored0 += 2 * U.shape[1] * max(i1 - i0, j1 - j0)
+    info = dict(stored_sigma=int(stored0), max_rank_sigma=int(ranks0.max()) if nB > 1 else 0,
+                ranks_sigma=ranks0)
+    L_diag, L_off = [None] * nB, {}
+    for k in range(nB):
+        try:
+            Lkk = cholesky(A_diag[k], lower=True, check_finite=False)
+        except LinAlgError:
+            info.update(completed=False, failed_at=k)
+            return None, None, info
+        L_diag[k] = Lkk
+        # panel: L_ik = U (L_kk^{-1} V)^T (17:179-185)
+        for i in range(k + 1, nB):
+            U, V = A_off.pop((i, k))
+            if U.shape[1]:
+                V = solve_triangular(Lkk, V, lower=True, check_finite=False)
+            L_off[(i, k)] = (U, V)
+        # trailing update A_ij -= L_ik L_jk^T for k < j <= i (17:187-205)
+        for i in range(k + 1, nB):
+            Ui, Vi = L_off[(i, k)]
+            if Ui.shape[1] == 0:
+                continue
+            for j in range(k + 1, i + 1):
+                Uj, Vj = L_off[(j, k)]
+                if Uj.shape[1] == 0:
+                    continue
+                W = Vi.T @ Vj
+                if i == j:
+                    A_diag[i] -= (Ui @ W) @ Ui.T
+                else:
+                    Uij, Vij = A_off[(i, j)]
+                    Ur, Vr, _ = recompress(np.hstack([Uij, -(Ui @ W)]), np.hstack([Vij, Uj]),
+                                           rule, thr_rel, thr_abs)
+                    A_off[(i, j)] = (Ur, Vr)
+    ranksF = np.zeros((nB, nB), np.int64)
+    for (i, j), (U, _) in L_off.items():
+        ranksF[i, j] = U.shape[1]
+    stored = sum(d.shape[0] ** 2 for d in L_diag) + sum(2 * u.shape[1] * u.shape[0] for u, _ in L_off.values())
+    rk = [u.shape[1] for u, _ in L_off.values()]
+    info.update(completed=True, stored_factor=int(stored),
+                mean_rank_factor=float(np.mean(rk)) if rk else 0.0,
+                max_rank_factor=int(max(rk)) if rk else 0, ranks_factor=ranksF)
+    return L_diag, L_off, info
+
+
+def tlr_forward(L_diag, L_off, bounds, B):
+    """Solve L Z = B by tile forward substitution.  Follows 17:230-240."""
+    Z = np.zeros_like(B)
+    for k, (k0, k1) in enumerate(bounds):
+        rhs = B[k0:k1].copy()
+        for j, (j0, j1) in enumerate(bounds[:k]):
+            U, V = L_off[(k, j)]
+            if U.shape[1]:
+                rhs -= U @ (V.T @ Z[j0:j1])
+        Z[k0:k1] = solve_triangular(L_diag[k], rhs, lower=True, check_finite=False)
+    return Z
+
+
+def tlr_backward(L_diag, L_off, bounds, Z):
+    """Solve L^T X = Z by tile backward substitution (L_ij^T = V U^T).  Follows 17:243-256."""
+    nB = len(bounds)
+    X = np.zeros_like(Z)
+    for k in range(nB - 1, -1, -1):
+        k0, k1 = bounds[k]
+        rhs = Z[k0:k1].copy()
+        for i in range(k + 1, nB):
+            i0, i1 = bounds[i]
+            U, V = L_off[(i, k)]
+            if U.shape[1]:
+                rhs -= V @ (U.T @ X[i0:i1])
+        X[k0:k1] = solve_triangular(L_diag[k], rhs, lower=True, trans="T", check_finite=False)
+    return X
+
+
+def tlr_matvec_L(L_diag, L_off, bounds, Z):
+    """Y = L Z for the tile factor.  Follows 17:218-227."""
+    Y = np.zeros_like(Z)
+    for k, (k0, k1) in enumerate(bounds):
+        Y[k0:k1] = L_diag[k] @ Z[k0:k1]
+        for j, (j0, j1) in enumerate(bounds[:k]):
+            U, V = L_off[(k, j)]
+            if U.shape[1]:
+                Y[k0:k1] += U @ (V.T @ Z[j0:j1])
+    return Y
+
+
+def tlr_loglik(S, y, theta, bounds, rule, value):
+    """TLR log-likelihood; (ll, (L_diag, L_off), info) or (None, None, info).  Follows 17:259-266.
+    info additionally carries 'logdet' and 'quad' (z^T z)."""
+    L_diag, L_off, info = tlr_cholesky(S, theta, bounds, rule, value)
+    if L_diag is None:
+        return None, None, info
+    z = tlr_forward(L_diag, L_off, bounds, y[:, None])[:, 0]
+    logdet = 2.0 * sum(np.log(np.diag(d)).sum() for d in L_diag)
+    quad = float(z @ z)
+    info.update(logdet=float(logdet), quad=quad)
+    return float(-0.5 * (quad + logdet + len(y) * LOG2PI)), (L_diag, L_off), info
+
+
+def tlr_predict(S, y, Sp, theta, bounds, fac):
+    """Kriging mean and sd at Sp through the TLR solves.  Follows 17:269-276."""
+    L_diag, L_off = fac
+    Kp = theta[0] * matern32(dist(Sp, S), theta[1])
+    alpha = tlr_backward(L_diag, L_off, bounds, tlr_forward(L_diag, L_off, bounds, y[:, None]))[:, 0]
+    mu = Kp @ alpha
+    W = tlr_forward(L_diag, L_off, bounds, Kp.T)
+    var = theta[0] * (1 + theta[2]) - (W ** 2).sum(0)
+    return mu, np.sqrt(np.maximum(var, 1e-12))
+
+
+def flops_factorization(ranks0, ranksF, b, nB):
+    """Algorithmic flop count of one TLR factorization + forward solve from realised ranks.
+
+    POTRF b^3/3 per diagonal tile; TRSM b^2 r per panel tile; diagonal SYRK 2 b r^2 + 2 b^2 r; every
+    recompression 4 b r_ik r_jk + 8 b w^2 + 22 w^3 + 4 b w r' with w = r_ij + r_jk (the final rank
+    stands in for the intermediate r'); forward solve nB b^2 + 2 sum b r."""
+    f = nB * b ** 3 / 3.0
+    rF = ranksF
+    for k in range(nB):
+        col = rF[k + 1:, k]
+        f += (b * b * col).sum() + (2 * b * col ** 2 + 2 * b * b * col).sum()
+        for i in range(k + 1, nB):
+            for j in range(k + 1, i):
+                if rF[i, k] and rF[j, k]:
+                    w = rF[i, j] + rF[j, k]
+                    f += 4 * b * rF[i, k] * rF[j, k] + 8 * b * w * w + 22 * w ** 3 + 4 * b * w * rF[i, j]
+    f += nB * b * b + 2 * b * rF.sum()
+    return float(f)

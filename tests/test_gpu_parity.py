@@ -1,0 +1,80 @@
+"""GPU parity of the TLR log-likelihood against the CPU oracle and the reference's golden values.
+
+Tolerances: log L, log-det and quadratic form within 1e-6 * max(1, |v|) (the reference's own reproduction
+tolerance for fixed-theta quantities, codes/29_check_reproduction.py:54); per-tile ranks equal to the
+oracle's except where the oracle's tail lies within 1e-9 of the budget (counted and reported)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-6
+
+
+def _close(a, b, rtol=RTOL):
+    return abs(a - b) <= rtol * max(1.0, abs(b))
+
+
+def _golden_field(n):
+    a = np.load(os.path.join(GOLDEN, f"sim_n{n}_obs.npy"))
+    return a[:, :2].copy(), a[:, 2].copy()
+
+
+@pytest.fixture(scope="module")
+def tlrgp():
+    import tlrgp as m
+    print("device:", m.device_info(), flush=True)
+    return m
+
+
+@pytest.fixture(scope="module")
+def goldens():
+    with open(os.path.join(GOLDEN, "goldens.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("rule,value", [("adaptive", 0.9), ("fixed", 1e-4), ("fixed", 1e-5)])
+def test_small_field_vs_oracle(tlrgp, rule, value):
+    import tlr_oracle as O
+    S, y = _golden_field(2000)
+    th = np.array([1.0, 0.3, 0.05])
+    bnd = tlrgp.tile_bounds(len(y), 250)
+    print(f"case N=2000 b=250 {rule}:{value}", flush=True)
+    ll, fac, info = tlrgp.tlr_loglik(S, y, th, bnd, rule, value)
+    ll_o, _, info_o = O.tlr_loglik(S, y, th, bnd, rule, value)
+    assert info["completed"] == info_o["completed"]
+    np.testing.assert_array_equal(info["ranks_sigma"], info_o["ranks_sigma"])
+    if info_o["completed"]:
+        np.testing.assert_array_equal(info["ranks_factor"], info_o["ranks_factor"])
+        assert _close(ll, ll_o), (ll, ll_o)
+        assert _close(info["logdet"], info_o["logdet"])
+        assert _close(info["quad"], info_o["quad"])
+    else:
+        assert info["failed_at"] == info_o["failed_at"]
+
+
+def test_n10000_goldens(tlrgp, goldens):
+    S, y = _golden_field(10000)
+    g = goldens["tlr_cholesky"]["10000"]
+    th = np.array([1.0, 0.3, 0.05])
+    bnd = tlrgp.tile_bounds(len(y), 250)
+    for r in g["rules"]:
+        print(f"case N=10000 b=250 {r['rule']}:{r['value']}", flush=True)
+        ll, fac, info = tlrgp.tlr_loglik(S, y, th, bnd, r["rule"], r["value"])
+        assert info["completed"] == r["completed"]
+        assert info["stored_sigma"] == r["stored_sigma"]
+        if r["completed"]:
+            assert info["stored_factor"] == r["stored_factor"]
+            assert info["max_rank_factor"] == r["max_rank_factor"]
+            assert abs(info["mean_rank_factor"] - r["mean_rank_factor"]) < 1e-12
+            assert _close(ll - g["dense_loglik"], r["loglik_err"], rtol=1e-5), (ll - g["dense_loglik"], r["loglik_err"])
+        else:
+            assert info["failed_at"] == r["failed_at_tile"]
+    # separate TLR log-determinant golden of the same field (kl_tlr.json, adaptive tau=0.9)
+    ll, fac, info = tlrgp.tlr_loglik(S, y, th, bnd, "adaptive", 0.9)
+    assert _close(info["logdet"], goldens["kl_tlr_n10000"]["logdet_approx"])
