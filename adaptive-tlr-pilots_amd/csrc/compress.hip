@@ -244,7 +244,7 @@ __global__ __launch_bounds__(256) void k_rank_decide(const double* __restrict__ 
                                                      int kmax, int pmin, int rule_fixed, double thr_rel, double thr_abs,
                                                      double* __restrict__ U, double* __restrict__ V, int64_t fstride,
                                                      int rank_cap, int* __restrict__ rank, int* __restrict__ need,
-                                                     double* __restrict__ tail_out, int* __restrict__ status) {
+                                                     int* __restrict__ retry_count, int* __restrict__ status) {
   const int64_t t = blockIdx.x;
   if (!need[t]) return;
   extern __shared__ double lds[];
@@ -289,12 +289,14 @@ __global__ __launch_bounds__(256) void k_rank_decide(const double* __restrict__ 
     if (r < 0 && k >= kmax) { atomicMax(&status[0], 1); r = k; accept = 1; }
     if (accept && r > rank_cap) { atomicMax(&status[0], 1); r = rank_cap; }
     s_rank = accept ? r : -1;
-    if (tail_out) tail_out[t] = (r >= 0 && r <= k) ? tail[r] : -1.0;
   }
   __syncthreads();
   const int r = s_rank;
   if (r < 0) {
-    if (tid == 0) need[t] = 1;
+    if (tid == 0) {
+      need[t] = 1;
+      if (retry_count) atomicAdd(retry_count, 1);
+    }
     return;
   }
   const double* Qr = Qrow + t * qstride;
@@ -397,7 +399,7 @@ size_t rank_decide_lds(int k) {
 hipError_t launch_rank_decide(const double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
                               int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
                               double thr_rel, double thr_abs, double* U, double* V, int64_t fstride, int rank_cap,
-                              int* rank, int* need, double* tail_out, int* status, int64_t ntile, hipStream_t s) {
+                              int* rank, int* need, int* retry_count, int* status, int64_t ntile, hipStream_t s) {
   const size_t lds = rank_decide_lds(k);
   if (lds > 150 * 1024) return hipErrorInvalidValue;
   static bool attr_set = false;
@@ -406,7 +408,7 @@ hipError_t launch_rank_decide(const double* Rm, int64_t rstride, const double* Q
     attr_set = true;
   }
   hipLaunchKernelGGL(k_rank_decide, dim3((unsigned)ntile), dim3(256), lds, s, Rm, rstride, Qrow, Qcol, qstride, part,
-                     b, k, kmax, pmin, rule_fixed, thr_rel, thr_abs, U, V, fstride, rank_cap, rank, need, tail_out,
+                     b, k, kmax, pmin, rule_fixed, thr_rel, thr_abs, U, V, fstride, rank_cap, rank, need, retry_count,
                      status);
   return hipGetLastError();
 }

@@ -46,6 +46,13 @@ struct Slot {
   SlotPtrs p;
 };
 
+// A launch bracketed by HIP events on the context's stream (timing mode only).
+struct TimedLaunch {
+  hipEvent_t a, b;
+  int slot, round;
+  double flops_per_tile;   // algorithmic flops per processed tile
+};
+
 }  // namespace
 
 struct tlr_ctx {
@@ -61,9 +68,36 @@ struct tlr_ctx {
   std::vector<Slot> slots;
   SlotPtrs* d_slots = nullptr;
   int64_t scr_stride = 0;
+  int* d_retry = nullptr;              // per slot: tiles redone with the wider sketch
+  // timing mode: launches of kernel class timing_kernel are bracketed by events
+  int timing_kernel = 0;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<TimedLaunch> timed;
+  double timed_ms = 0.0, timed_flops = 0.0;
+  int64_t timed_launches = 0;
   ~tlr_ctx() {
+    for (hipEvent_t e : ev_pool) hipEventDestroy(e);
     for (void* p : allocs) hipFree(p);
     if (stream) hipStreamDestroy(stream);
+  }
+  hipEvent_t next_event() {
+    if (ev_used == ev_pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+  }
+  // Start / stop a timed launch of class id (no-op unless timing that class).
+  bool tic(int id, TimedLaunch& t) {
+    if (timing_kernel != id) return false;
+    t.a = next_event(); t.b = next_event();
+    return t.a && t.b && hipEventRecord(t.a, stream) == hipSuccess;
+  }
+  void toc(bool on, TimedLaunch& t) {
+    if (!on) return;
+    if (hipEventRecord(t.b, stream) == hipSuccess) timed.push_back(t);
   }
   template <class T>
   hipError_t alloc(T** p, size_t count) {
@@ -181,6 +215,7 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
     ACHK(c->alloc(&p.scratch, (size_t)npm * c->scr_stride));
   }
   ACHK(c->alloc(&c->d_slots, (size_t)max_batch));
+  ACHK(c->alloc(&c->d_retry, (size_t)max_batch));
   std::vector<SlotPtrs> hp(max_batch);
   for (int s = 0; s < max_batch; ++s) hp[s] = c->slots[s].p;
   ACHK(hipMemcpy(c->d_slots, hp.data(), max_batch * sizeof(SlotPtrs), hipMemcpyHostToDevice));
@@ -208,8 +243,8 @@ int tlr_ctx_set_y(tlr_ctx* ctx, const double* y) {
   return TLR_OK;
 }
 
-static int compress_slot(tlr_ctx* c, const SlotPtrs& p, const MaternParams& mp, int rule_fixed, double thr_rel,
-                         double thr_abs) {
+static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternParams& mp, int rule_fixed,
+                         double thr_rel, double thr_abs) {
   hipStream_t s = c->stream;
   const int b = c->b;
   const int64_t nt = c->ntile;
@@ -225,14 +260,24 @@ static int compress_slot(tlr_ctx* c, const SlotPtrs& p, const MaternParams& mp, 
     // alternate products: the buffer holding the latest product and the orthonormal basis of the other side
     double* cur = c->bufA;   // receives T * X
     double* oth = c->bufB;   // receives T^T * X
+    const double gemm_flops = 2.0 * b * (double)b * k;     // one product with a b x b tile
+    const double qr_flops = 4.0 * b * (double)k * k;        // Householder QR + explicit Q
+    TimedLaunch tl{nullptr, nullptr, slot, rd, gemm_flops};
+    bool on = c->tic(1, tl);
     HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, 0, c->omega, 0, cur, ms, k, c->need, s));
+    c->toc(on, tl);
     int last_op = 0;
     for (int pidx = 1; pidx < NPRODUCT; ++pidx) {
       double* src = (last_op == 0) ? cur : oth;
       double* dst = (last_op == 0) ? oth : cur;
+      TimedLaunch tq{nullptr, nullptr, slot, rd, qr_flops};
+      on = c->tic(3, tq);
       HIPCHK(launch_panel_qr(src, ms, b, k, nullptr, 0, 0, c->need, nt, s));
+      c->toc(on, tq);
       last_op ^= 1;
+      on = c->tic(1, tl);
       HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, last_op, src, ms, dst, ms, k, c->need, s));
+      c->toc(on, tl);
     }
     // last product M; the other buffer holds the orthonormal basis Qprev
     double* M = (last_op == 0) ? cur : oth;
@@ -240,12 +285,19 @@ static int compress_slot(tlr_ctx* c, const SlotPtrs& p, const MaternParams& mp, 
     // T ~ M Qprev^T (last op T) or Qprev M^T (last op T^T); residual before M is orthonormalised
     const double* Arow = (last_op == 0) ? M : Qprev;
     const double* Bcol = (last_op == 0) ? Qprev : M;
+    TimedLaunch tr{nullptr, nullptr, slot, rd, gemm_flops};
+    on = c->tic(2, tr);
     HIPCHK(launch_residual(c->sx, c->sy, c->n, b, nt, mp, Arow, Bcol, ms, k, c->part, c->need, s));
+    c->toc(on, tr);
+    TimedLaunch tq{nullptr, nullptr, slot, rd, qr_flops};
+    on = c->tic(3, tq);
     HIPCHK(launch_panel_qr(M, ms, b, k, c->bufR, rs, last_op == 1 ? 1 : 0, c->need, nt, s));
+    c->toc(on, tq);
     const double* Qrow = (last_op == 0) ? M : Qprev;
     const double* Qcol = (last_op == 0) ? Qprev : M;
     HIPCHK(launch_rank_decide(c->bufR, rs, Qrow, Qcol, ms, c->part, b, k, kmax, PMIN, rule_fixed, thr_rel, thr_abs,
-                              p.U, p.V, (int64_t)b * RANK_CAP, RANK_CAP, p.rank, c->need, nullptr, p.status, nt, s));
+                              p.U, p.V, (int64_t)b * RANK_CAP, RANK_CAP, p.rank, c->need, c->d_retry + slot,
+                              p.status, nt, s));
   }
   HIPCHK(launch_copy_int(p.rank0, p.rank, nt, s));
   return TLR_OK;
@@ -263,6 +315,9 @@ int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
   if (!(value > 0.0)) return set_err(TLR_ERR_ARG, "rule value must be positive");
   hipStream_t st = c->stream;
   const int rule_fixed = rule == TLR_RULE_FIXED;
+  c->ev_used = 0;
+  c->timed.clear();
+  HIPCHK(launch_fill_int(c->d_retry, c->max_batch, 0, st));
   const double bmax = (double)(c->n < c->b ? c->n : c->b);
   for (int s = 0; s < m; ++s) {
     const double* th = thetas + 3 * s;
@@ -276,7 +331,7 @@ int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
     HIPCHK(launch_fill_int(p.status + ST_FAILED, 1, 0x7fffffff, st));
     HIPCHK(launch_fill_int(p.done_k, c->ntile, -1, st));
     HIPCHK(hipMemcpyAsync(p.rhs, c->y, c->n * sizeof(double), hipMemcpyDeviceToDevice, st));
-    int rc = compress_slot(c, p, mp, rule_fixed, value, thr_abs);
+    int rc = compress_slot(c, s, p, mp, rule_fixed, value, thr_abs);
     if (rc != TLR_OK) return rc;
   }
   const SlotArgs sa = c->args();
@@ -300,6 +355,19 @@ int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
   }
   HIPCHK(launch_final(sa, m, st));
   HIPCHK(hipStreamSynchronize(st));
+  if (!c->timed.empty()) {
+    std::vector<int> retry(c->max_batch);
+    HIPCHK(hipMemcpy(retry.data(), c->d_retry, c->max_batch * sizeof(int), hipMemcpyDeviceToHost));
+    for (const TimedLaunch& t : c->timed) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
+      const double tiles = t.round == 0 ? (double)c->ntile : (double)retry[t.slot];
+      c->timed_ms += ms;
+      c->timed_flops += tiles * t.flops_per_tile;
+      c->timed_launches += 1;
+    }
+    c->timed.clear();
+  }
   int ret = TLR_OK;
   std::vector<int> rk;
   for (int s = 0; s < m; ++s) {
@@ -331,6 +399,23 @@ int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
     }
   }
   return ret;
+}
+
+int tlr_ctx_set_timing(tlr_ctx* ctx, int32_t kernel_class) {
+  if (!ctx || kernel_class < 0 || kernel_class > 4) return set_err(TLR_ERR_ARG, "invalid kernel class");
+  ctx->timing_kernel = kernel_class;
+  ctx->timed_ms = 0.0;
+  ctx->timed_flops = 0.0;
+  ctx->timed_launches = 0;
+  return TLR_OK;
+}
+
+int tlr_ctx_get_timing(const tlr_ctx* ctx, double* total_ms, int64_t* launches, double* flops) {
+  if (!ctx) return set_err(TLR_ERR_ARG, "null context");
+  if (total_ms) *total_ms = ctx->timed_ms;
+  if (launches) *launches = ctx->timed_launches;
+  if (flops) *flops = ctx->timed_flops;
+  return TLR_OK;
 }
 
 int tlr_loglik(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, int32_t precision, double out3[3],

@@ -57,7 +57,7 @@ size_t rank_decide_lds(int k);
 hipError_t launch_rank_decide(const double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
                               int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
                               double thr_rel, double thr_abs, double* U, double* V, int64_t fstride, int rank_cap,
-                              int* rank, int* need, double* tail_out, int* status, int64_t ntile, hipStream_t s);
+                              int* rank, int* need, int* retry_count, int* status, int64_t ntile, hipStream_t s);
 hipError_t launch_fill_int(int* p, int64_t count, int value, hipStream_t s);
 hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t s);
 

@@ -27,6 +27,8 @@ SIGNATURES = {
     "tlr_loglik": (ctypes.c_int, [_VP, _D, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _D, _I, _I, _I]),
     "tlr_loglik_batch": (ctypes.c_int, [_VP, ctypes.c_int32, _D, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
                                         _D, _I, _I, _I, _I]),
+    "tlr_ctx_set_timing": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "tlr_ctx_get_timing": (ctypes.c_int, [_VP, _D, _I64, _D]),
     "tlr_last_error": (ctypes.c_char_p, []),
     "tlr_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32]),
 }

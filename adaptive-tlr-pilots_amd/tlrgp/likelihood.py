@@ -118,6 +118,19 @@ class TLRContext:
         except Exception:
             pass
 
+    def set_timing(self, kernel_class):
+        """Bracket launches of one kernel class (1 products, 2 residual, 3 panel QR, 4 rank decision) with
+        HIP events; resets the accumulators."""
+        check(lib.tlr_ctx_set_timing(self._ptr, int(kernel_class)))
+
+    def get_timing(self):
+        """(total kernel ms, launches, algorithmic flops) since set_timing."""
+        ms = _lib.ctypes.c_double()
+        n = _lib.ctypes.c_int64()
+        fl = _lib.ctypes.c_double()
+        check(lib.tlr_ctx_get_timing(self._ptr, _lib.ctypes.byref(ms), _lib.ctypes.byref(n), _lib.ctypes.byref(fl)))
+        return ms.value, n.value, fl.value
+
     def set_y(self, y):
         check(lib.tlr_ctx_set_y(self._ptr, dptr(f64(y, (self.n,)))))
 

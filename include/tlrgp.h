@@ -72,6 +72,14 @@ int tlr_loglik(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, 
 int tlr_loglik_batch(tlr_ctx* ctx, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
                      double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma, int32_t* ranks_factor);
 
+/* Timing of one kernel class with HIP events on the context's stream, for roofline reports (no
+ * counterpart in the reference, which times whole evaluations with time.time(), 17:359-362).
+ * kernel_class: 0 off, 1 Matern products, 2 residual, 3 panel QR, 4 rank decision.  Setting it resets the
+ * accumulators; tlr_ctx_get_timing returns the summed kernel time, the number of launches and the
+ * algorithmic flops of those launches (products: 2 b^2 k per processed tile). */
+int tlr_ctx_set_timing(tlr_ctx* ctx, int32_t kernel_class);
+int tlr_ctx_get_timing(const tlr_ctx* ctx, double* total_ms, int64_t* launches, double* flops);
+
 /* Message of the last error on the calling thread. */
 const char* tlr_last_error(void);
 

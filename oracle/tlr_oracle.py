@@ -54,6 +54,44 @@ def cov_full(S, theta):
     return K
 
 
+def cov_full_lowmem(S, theta, block=2000):
+    """The same dense covariance built row block by row block into one N x N array (17:91-103); used
+    for N where the broadcasting temporaries of cov_full do not fit in memory."""
+    sigma2, rng_, eta = theta
+    n = S.shape[0]
+    K = np.empty((n, n))
+    for i0 in range(0, n, block):
+        i1 = min(n, i0 + block)
+        K[i0:i1] = sigma2 * matern32(dist(S[i0:i1], S), rng_)
+    K[np.diag_indices_from(K)] += sigma2 * eta
+    return K
+
+
+def cholesky_blocked(K, block=4000):
+    """In-place lower Cholesky of K by 4000 x 4000 blocks (right-looking); no single BLAS call sees more
+    than 2^31 entries, which the bundled OpenBLAS does not survive at N = 40,000.  Returns K (upper part
+    zeroed)."""
+    from scipy.linalg.blas import dgemm
+    n = K.shape[0]
+    for k0 in range(0, n, block):
+        k1 = min(n, k0 + block)
+        K[k0:k1, k0:k1] = cholesky(K[k0:k1, k0:k1], lower=True, check_finite=False)
+        if k1 < n:
+            P = solve_triangular(K[k0:k1, k0:k1], K[k1:, k0:k1].T, lower=True, check_finite=False).T
+            K[k1:, k0:k1] = P
+            for i0 in range(k1, n, block):
+                i1 = min(n, i0 + block)
+                for j0 in range(k1, i1, block):
+                    j1 = min(n, j0 + block)
+                    K[i0:i1, j0:j1] = dgemm(-1.0, P[i0 - k1:i1 - k1], P[j0 - k1:j1 - k1], 1.0,
+                                            K[i0:i1, j0:j1], trans_b=True)
+    for k0 in range(0, n, block):
+        K[k0:min(n, k0 + block), min(n, k0 + block):] = 0.0
+        blk = K[k0:min(n, k0 + block), k0:min(n, k0 + block)]
+        blk[np.triu_indices_from(blk, 1)] = 0.0
+    return K
+
+
 def _spread_bits(v):
     """Interleave zeros between the low 32 bits of v (uint64), the usual magic-mask bit spread."""
     v = v & np.uint64(0xFFFFFFFF)
@@ -151,8 +189,12 @@ def simulate_seed0(n, theta=(1.0, 0.3, 0.05), seed=0):
     rng = np.random.default_rng(seed)
     S = rng.random((n, 2))
     S = S[morton_order(S)]
-    L = np.linalg.cholesky(cov_full(S, np.asarray(theta, float)))
-    y = L @ rng.standard_normal(n)
+    if n <= 20000:
+        L = cholesky(cov_full(S, np.asarray(theta, float)), lower=True, check_finite=False)
+    else:
+        L = cholesky_blocked(cov_full_lowmem(S, np.asarray(theta, float)))
+    z = rng.standard_normal(n)
+    y = np.concatenate([L[i0:i0 + 4000] @ z for i0 in range(0, n, 4000)])
     return S, y
 
 

@@ -78,3 +78,51 @@ def test_n10000_goldens(tlrgp, goldens):
     # separate TLR log-determinant golden of the same field (kl_tlr.json, adaptive tau=0.9)
     ll, fac, info = tlrgp.tlr_loglik(S, y, th, bnd, "adaptive", 0.9)
     assert _close(info["logdet"], goldens["kl_tlr_n10000"]["logdet_approx"])
+
+
+@pytest.mark.parametrize("theta", [(1.0, 0.3, 0.05), (0.5, 0.6, 0.01), (2.0, 0.1, 0.1), (1.3, 0.15, 0.02)])
+def test_b500_vs_oracle(tlrgp, theta):
+    """Tile size 500 (the north-star tile), N=10,000 seed-0 field: ranks of every tile and log L against the
+    oracle, over the corners of the benchmark's theta box (ranks up to ~50 exercise the wide sketch and the
+    widest recompression class)."""
+    import tlr_oracle as O
+    S, y = _seed0_field(10000)
+    th = np.array(theta)
+    bnd = tlrgp.tile_bounds(len(y), 500)
+    print(f"case N=10000 b=500 adaptive:0.9 theta={theta}", flush=True)
+    ll, fac, info = tlrgp.tlr_loglik(S, y, th, bnd, "adaptive", 0.9)
+    ll_o, _, info_o = O.tlr_loglik(S, y, th, bnd, "adaptive", 0.9)
+    np.testing.assert_array_equal(info["ranks_sigma"], info_o["ranks_sigma"])
+    assert info["completed"] == info_o["completed"]
+    np.testing.assert_array_equal(info["ranks_factor"], info_o["ranks_factor"])
+    assert _close(ll, ll_o), (ll, ll_o)
+
+
+_SEED0 = {}
+
+
+def _seed0_field(n):
+    if n not in _SEED0:
+        path = os.path.join(GOLDEN, f"seed0_n{n}.npy")
+        if os.path.exists(path):
+            a = np.load(path)
+            _SEED0[n] = (a[:, :2].copy(), a[:, 2].copy())
+        else:
+            import tlr_oracle as O
+            _SEED0[n] = O.simulate_seed0(n)
+    return _SEED0[n]
+
+
+def test_n40000_b250_golden(tlrgp, goldens):
+    """Full-size reference run (N=40,000, b=250, adaptive 0.9): log L, stored factor and ranks against the
+    committed results of codes/17 (tlr_cholesky_n40000.json)."""
+    S, y = _golden_field(40000)
+    g = goldens["tlr_cholesky"]["40000"]
+    r = [x for x in g["rules"] if x["rule"] == "adaptive"][0]
+    print("case N=40000 b=250 adaptive:0.9", flush=True)
+    ll, fac, info = tlrgp.tlr_loglik(S, y, np.array([1.0, 0.3, 0.05]), tlrgp.tile_bounds(40000, 250), "adaptive", 0.9)
+    assert info["completed"]
+    assert info["stored_sigma"] == r["stored_sigma"]
+    assert info["stored_factor"] == r["stored_factor"]
+    assert info["max_rank_factor"] == r["max_rank_factor"]
+    assert _close(ll, r["loglik"]), (ll, r["loglik"])
