@@ -233,11 +233,6 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ sx,
 }
 
 // ------------------------------------------------------------------------------ rank decision
-// For tile t: core = R (k x k, already transposed by the QR kernel when the last product was with T^T),
-// one-sided Jacobi core J = G; tails from the verified residual; rank by the rule; outputs
-// U = Qrow G[:, perm[:r]], V = Qcol J[:, perm[:r]] into the factor slots.
-// need[t] <- 0 when accepted, 1 when a wider sketch is needed.  status[0] (rank cap) is raised with
-// an atomic max when the rank exceeds the slot capacity.
 __global__ __launch_bounds__(256) void k_rank_decide(const double* __restrict__ Rm, int64_t rstride,
                                                      const double* __restrict__ Qrow, const double* __restrict__ Qcol,
                                                      int64_t qstride, const double* __restrict__ part, int b, int k,
@@ -269,7 +264,10 @@ __global__ __launch_bounds__(256) void k_rank_decide(const double* __restrict__ 
     s_res2 = r2; s_fro2 = f2;
   }
   __syncthreads();
-  jacobi_svd<4>(G, k, J, k, k, sig, perm, &flag, 60);
+  if (jacobi_svd<4>(G, k, J, k, k, sig, perm, &flag, 60) == -2) {   // non-finite tile: reported, not stored
+    if (tid == 0) { atomicMax(&status[ST_GUARD], 2); need[t] = 0; }
+    return;
+  }
   if (tid == 0) {
     // tail[r] = sqrt(res2 + sum_{i >= r} sig_perm[i]^2), r = 0..k
     double acc = s_res2;
@@ -401,10 +399,10 @@ hipError_t launch_rank_decide(const double* Rm, int64_t rstride, const double* Q
                               double thr_rel, double thr_abs, double* U, double* V, int64_t fstride, int rank_cap,
                               int* rank, int* need, int* retry_count, int* status, int64_t ntile, hipStream_t s) {
   const size_t lds = rank_decide_lds(k);
-  if (lds > 150 * 1024) return hipErrorInvalidValue;
+  if (lds > 158 * 1024) return hipErrorInvalidValue;
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void*)k_rank_decide, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_rank_decide, hipFuncAttributeMaxDynamicSharedMemorySize, 158 * 1024);
     attr_set = true;
   }
   hipLaunchKernelGGL(k_rank_decide, dim3((unsigned)ntile), dim3(256), lds, s, Rm, rstride, Qrow, Qcol, qstride, part,

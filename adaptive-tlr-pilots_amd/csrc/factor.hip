@@ -196,33 +196,60 @@ __global__ __launch_bounds__(256) void k_trsm(SlotArgs sa, int k) {
     r = P.rank[tik];
     X = P.V + tik * sa.fstride;
   }
-  if (r == 0) return;
+  if (r <= 0 || r > rc) {
+    if (r != 0) atomicMax(&P.status[ST_GUARD], 1);
+    return;
+  }
   const int bk = (int)min<int64_t>(b, sa.n - (int64_t)k * b);
   const int brows = (item == 0) ? bk : b;   // rhs holds exactly the tile's rows
-  __shared__ double tmp[NB * 64];
+  __shared__ double Lb[NB][NB + 1];         // L[p0 + ll][q0 + qq]
+  __shared__ double XT[2][NB][64];          // Xb: X[q0 + qq][c]; tmp: corrected rows of the panel
+  double (*Xb)[64] = XT[0];
+  double (*tmp)[64] = XT[1];
   __shared__ double Dl[NB][NB + 1];
   const int tid = threadIdx.x;
+  // thread -> (row ll, columns c = cg, cg + 4, ...): 32 rows x 8 column groups
+  const int ll = tid % NB, cg = tid / NB;
   for (int p0 = 0; p0 < brows; p0 += NB) {
     const int wd = min(NB, brows - p0);
-    for (int e = tid; e < NB * NB; e += 256) Dl[e % NB][e / NB] = Dv[(p0 / NB) * NB * NB + e];
-    for (int e = tid; e < NB * r; e += 256) {
-      const int ll = e % NB, c = e / NB;
-      double s = 0.0;
-      if (ll < wd) {
-        const int row = p0 + ll;
-        s = X[row + (int64_t)c * b];
-        double acc = 0.0;
-        for (int q = 0; q < p0; ++q) acc = fma(L[row + (int64_t)q * b], X[q + (int64_t)c * b], acc);
-        s -= acc;
+    double acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+    // X[p0:p0+32] -= L[p0:p0+32, 0:p0] X[0:p0], 32 x 32 blocks of L staged through LDS
+    for (int q0 = 0; q0 < p0; q0 += NB) {
+      for (int e = tid; e < NB * NB; e += 256) {
+        const int rr = e % NB, qq = e / NB;
+        Lb[rr][qq] = (rr < wd) ? L[(p0 + rr) + (int64_t)(q0 + qq) * b] : 0.0;
       }
-      tmp[ll + c * NB] = s;
+      for (int e = tid; e < NB * r; e += 256) {
+        const int qq = e % NB, c = e / NB;
+        Xb[qq][c] = X[(q0 + qq) + (int64_t)c * b];
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int qq = 0; qq < NB; ++qq) {
+        const double lv = Lb[ll][qq];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int c = cg + 8 * u;
+          if (c < r) acc[u] = fma(lv, Xb[qq][c], acc[u]);
+        }
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < NB * NB; e += 256) Dl[e % NB][e / NB] = Dv[(p0 / NB) * NB * NB + e];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int c = cg + 8 * u;
+      if (c < r) tmp[ll][c] = (ll < wd) ? X[(p0 + ll) + (int64_t)c * b] - acc[u] : 0.0;
     }
     __syncthreads();
-    for (int e = tid; e < NB * r; e += 256) {
-      const int ll = e % NB, c = e / NB;
-      if (ll < wd) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int c = cg + 8 * u;
+      if (c < r && ll < wd) {
         double s = 0.0;
-        for (int m = 0; m <= ll; ++m) s = fma(Dl[ll][m], tmp[m + c * NB], s);
+        for (int m = 0; m <= ll; ++m) s = fma(Dl[ll][m], tmp[m][c], s);
         X[(p0 + ll) + (int64_t)c * b] = s;
       }
     }
@@ -233,21 +260,32 @@ __global__ __launch_bounds__(256) void k_trsm(SlotArgs sa, int k) {
   double* Wi = P.Wd + (int64_t)(k + item) * rc * rc;
   double* Xi = P.Xd + (int64_t)(k + item) * b * rc;
   const double* U = P.U + tik * sa.fstride;
-  __shared__ double Ws[64 * 64];
+  __shared__ double Wsh[64 * 64];
   const int w = wave_id(), l = lane_id();
   for (int e = w; e < r * r; e += 4) {
     const int a = e % r, c = e / r;
+    if (a > c) continue;
     double s = 0.0;
     for (int p = l; p < b; p += 64) s = fma(X[p + (int64_t)a * b], X[p + (int64_t)c * b], s);
     s = wave_sum(s);
-    if (l == 0) { Ws[a + c * r] = s; Wi[a + c * r] = s; }
+    if (l == 0) { Wsh[a + c * r] = s; Wsh[c + a * r] = s; Wi[a + c * r] = s; Wi[c + a * r] = s; }
   }
   __syncthreads();
-  for (int e = tid; e < b * r; e += 256) {
-    const int p = e % b, c = e / b;
-    double s = 0.0;
-    for (int a = 0; a < r; ++a) s = fma(U[p + (int64_t)a * b], Ws[a + c * r], s);
-    Xi[p + (int64_t)c * b] = s;
+  // X_i = U W, rows staged through LDS 64 at a time (Xb and tmp form one 64 x 64 buffer)
+  double* Ub = &Xb[0][0];
+  for (int r0 = 0; r0 < b; r0 += 64) {
+    for (int e = tid; e < 64 * r; e += 256) {
+      const int rr = e % 64, a = e / 64, p = r0 + rr;
+      Ub[a * 64 + rr] = (p < b) ? U[p + (int64_t)a * b] : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < 64 * r; e += 256) {
+      const int rr = e % 64, c = e / 64, p = r0 + rr;
+      double s = 0.0;
+      for (int a = 0; a < r; ++a) s = fma(Ub[a * 64 + rr], Wsh[a + c * r], s);
+      if (p < b) Xi[p + (int64_t)c * b] = s;
+    }
+    __syncthreads();
   }
 }
 
@@ -262,7 +300,10 @@ __global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
   const int i = k + 1 + blockIdx.y;
   const int64_t tik = tri_index(i, k);
   const int r = P.rank[tik];
-  if (r == 0) return;
+  if (r <= 0 || r > sa.rank_cap) {
+    if (r != 0) atomicMax(&P.status[ST_GUARD], 1);
+    return;
+  }
   const double* U = P.U + tik * sa.fstride;
   const double* Xi = P.Xd + (int64_t)i * b * sa.rank_cap;
   double* A = P.D + (int64_t)i * b * b;
@@ -332,118 +373,134 @@ __global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
 }
 
 // ------------------------------------------------------------------------------ recompression
-// One work group per pair (i, j), k < j < i, whose width w = r_ij + r_jk lies in [wlo, whi].
-// Householder QR of both concatenations in registers, one-sided Jacobi SVD of the core R_u R_v^T,
-// truncation by the rule, U_ij = Q_u G[:, perm[:r]], V_ij = Q_v J[:, perm[:r]].
-template <int RPL, int NW, bool BIG>
-__global__ __launch_bounds__(64 * NW) void k_recompress(SlotArgs sa, int k, int wlo, int whi, int rule_fixed,
-                                                       double thr_rel, double thr_abs) {
-  constexpr int WMAX = 8 * NW;
+// One work group per pair (i, j), k < j < i, with r_ik, r_jk > 0 (17:187-205).  The reference forms
+// Unew = [U_ij, -U_ik W] and Vnew = [V_ij, U_jk] (W = V_ik^T V_jk), takes QR of both, the SVD of
+// R_u R_v^T and truncates by the rule (17:119-133).  The same factorization is built here from the
+// structure those outputs already have: V_ij has orthonormal columns and U_ij orthogonal columns with
+// norms sigma_a, so with Q1 = U_ij diag(1/sigma) and X = -U_ik W,
+//   Vnew = [V_ij, Q2v] [[I, Cv], [0, R2v]],  Cv = V_ij^T U_jk,      U_jk - V_ij Cv = Q2v R2v
+//   Unew = [Q1,   Q2u] [[D, Cu], [0, R2u]],  Cu = Q1^T X,           X - Q1 Cu      = Q2u R2u
+// (block Gram-Schmidt applied twice, Householder QR of the r_jk new columns only).  The core
+// R_u R_v^T (w x w, w = r_ij + r_jk) goes to the one-sided Jacobi SVD; the outputs are formed row block
+// by row block through LDS, in place.  The panel holds the r_jk new columns: variant NW=4 for
+// r_jk <= 32, NW=8 for r_jk <= 64; both variants see every pair and keep the ones of their class.
+template <int NW>
+struct RecompLayout {
+  static constexpr int R2MAX = 8 * NW;
+  static constexpr int WMAX = 64 + R2MAX;
+  static constexpr int WJ = 48;                                   // cores up to 48 x 48 run in LDS
+  static constexpr bool RR_LDS = WMAX * R2MAX <= 3072;
+  static constexpr int FRONT = (2 * 64 * R2MAX > 64 * WMAX) ? 2 * 64 * R2MAX : 64 * WMAX;
+  static constexpr int RR = RR_LDS ? WMAX * R2MAX : 0;
+  static constexpr int OFF_RV = FRONT, OFF_RU = OFF_RV + RR, OFF_SC = OFF_RU + RR, OFF_SG = OFF_SC + 64,
+                       OFF_G = OFF_SG + 64, OFF_J = OFF_G + WJ * WJ, TOTAL = OFF_J + WJ * WJ;
+};
+
+template <int RPL, int NW>
+__global__ __launch_bounds__(64 * NW) void k_recompress(SlotArgs sa, int k, int r2lo, int r2hi) {
+  using L = RecompLayout<NW>;
   const int slot = blockIdx.z;
   const SlotPtrs P = sa.slot(slot);
   if (failed(P.status)) return;
   const int b = sa.b;
-  // pairs (a, b'), 0 <= b' < a, enumerated as tri_index(a, b'); index 0 is (1, 0)
   int a_, b_;
-  tri_decode((int64_t)blockIdx.x, &a_, &b_);
+  tri_decode((int64_t)blockIdx.x, &a_, &b_);   // pairs (a, b'), 0 <= b' < a; index 0 is (1, 0)
   const int i = a_ + k + 1, j = b_ + k + 1;
   if (i >= sa.nB) return;
   const int64_t tij = tri_index(i, j), tik = tri_index(i, k), tjk = tri_index(j, k);
-  const int r1 = P.rank[tij], rik = P.rank[tik], rjk = P.rank[tjk];
-  if (rik == 0 || rjk == 0) return;
-  const int w = r1 + rjk;
-  if (w < wlo || w > whi) return;
-  // the three width classes run as separate launches: a pair handled by an earlier class this step
-  // has already changed r_ij and must not be seen again
-  if (P.done_k[tij] == k) return;
+  const int r1 = P.rank[tij], rik = P.rank[tik], r2 = P.rank[tjk];
+  if (rik == 0 || r2 == 0 || r2 < r2lo || r2 > r2hi) return;
+  if (r1 < 0 || r1 > sa.rank_cap || rik < 0 || rik > sa.rank_cap) {
+    if (threadIdx.x == 0) atomicMax(&P.status[ST_GUARD], 1);
+    return;
+  }
+  const int w = r1 + r2;
+  const double thr_rel = sa.thr[2 * slot], thr_abs = sa.thr[2 * slot + 1];
   extern __shared__ double lds[];
-  double* G = lds;
-  double* Wm = lds;                          // aliases G (and J) until the core is formed
+  double* Wm = lds;                           // rik x r2          (phases 1-3)
+  double* Ct = lds + 64 * L::R2MAX;           // r1 x r2 temporary (phases 2-3)
+  double* Sb = lds;                           // 64 x w staging     (phase 5, aliases the front)
   double* scr = P.scratch + (int64_t)blockIdx.x * sa.scr_stride;
-  double* Qs = scr;                          // b x WMAX explicit Q (V side, then U side)
-  double* J = BIG ? scr + (int64_t)b * 128 : G + WMAX * WMAX;
-  double* Ru = BIG ? J + 128 * 128 : J + WMAX * WMAX;
-  double* Rv = Ru + (BIG ? 128 * 128 : WMAX * WMAX);
+  double* Q2v = scr;                          // b x r2
+  double* RvR = L::RR_LDS ? lds + L::OFF_RV : scr + (int64_t)b * 64;            // w x r2: [Cv; R2v]
+  double* RuR = L::RR_LDS ? lds + L::OFF_RU : scr + (int64_t)b * 64 + 128 * 64; // w x r2: [Cu; R2u]
+  double* sc = lds + L::OFF_SC;               // 1 / sigma_a of U_ij
+  double* sg = lds + L::OFF_SG;               // sigma_a
+  const bool core_lds = w <= L::WJ;
+  double* G = core_lds ? lds + L::OFF_G : scr + (int64_t)b * 64 + 2 * 128 * 64;
+  double* J = core_lds ? lds + L::OFF_J : G + 128 * 128;
   __shared__ double vbuf[2 * 64 * RPL];
-  __shared__ double tau[WMAX];
-  __shared__ double sig[WMAX];
-  __shared__ int perm[WMAX];
+  __shared__ double tau[8 * NW];
+  __shared__ double sig[L::WMAX];
+  __shared__ int perm[L::WMAX];
   __shared__ int flag, s_rank;
-  const int tid = threadIdx.x, wv = wave_id(), ln = lane_id();
-  double* Uij = P.U + tij * sa.fstride;
-  double* Vij = P.V + tij * sa.fstride;
-  const double* Uik = P.U + tik * sa.fstride;
-  const double* Vik = P.V + tik * sa.fstride;
-  const double* Ujk = P.U + tjk * sa.fstride;
-  const double* Vjk = P.V + tjk * sa.fstride;
+  const int tid = threadIdx.x, NT = 64 * NW, wv = wave_id(), ln = lane_id();
+  const int64_t fs = sa.fstride;
+  double* Uij = P.U + tij * fs;
+  double* Vij = P.V + tij * fs;
+  const double* Uik = P.U + tik * fs;
+  const double* Vik = P.V + tik * fs;
+  const double* Ujk = P.U + tjk * fs;
+  const double* Vjk = P.V + tjk * fs;
   RegPanel<RPL, NW> pan;
-  // ---- V side: [V_ij, U_jk]
-#pragma unroll
-  for (int jj = 0; jj < 8; ++jj) {
-    const int c = wv + NW * jj;
-#pragma unroll
-    for (int t = 0; t < RPL; ++t) {
-      const int p = ln + 64 * t;
-      double v = 0.0;
-      if (p < b && c < w) v = (c < r1) ? Vij[p + (int64_t)c * b] : Ujk[p + (int64_t)(c - r1) * b];
-      pan.a[jj][t] = v;
-    }
+
+  // ---- 1. W = V_ik^T V_jk
+  pan.load(Vjk, b, b, r2);
+  panel_coef(pan, Vik, b, b, rik, r2, (const double*)nullptr, Wm, rik);
+  // ---- 2. V side: [V_ij, U_jk] = [V_ij, Q2v] [[I, Cv], [0, R2v]]
+  pan.load(Ujk, b, b, r2);
+  if (r1 > 0) {
+    panel_coef(pan, Vij, b, b, r1, r2, (const double*)nullptr, RvR, w);
+    __syncthreads();
+    panel_axpy(pan, Vij, b, b, r1, r2, (const double*)nullptr, RvR, w, -1.0);
+    panel_coef(pan, Vij, b, b, r1, r2, (const double*)nullptr, Ct, r1);
+    __syncthreads();
+    panel_axpy(pan, Vij, b, b, r1, r2, (const double*)nullptr, Ct, r1, -1.0);
+    for (int e = tid; e < r1 * r2; e += NT) RvR[(e % r1) + (e / r1) * w] += Ct[e];
   }
-  pan.geqr2(b, w, vbuf, tau);
-  pan.store_r(Rv, w, w);
-  pan.org2r(b, w, vbuf, tau);
-  pan.store(Qs, b, b, w);
-  // ---- W = V_ik^T V_jk
-  for (int e = wv; e < rik * rjk; e += NW) {
-    const int a = e % rik, c = e / rik;
-    double s = 0.0;
-    for (int p = ln; p < b; p += 64) s = fma(Vik[p + (int64_t)a * b], Vjk[p + (int64_t)c * b], s);
-    s = wave_sum(s);
-    if (ln == 0) Wm[a + c * rik] = s;
+  pan.geqr2(b, r2, vbuf, tau);
+  pan.store_r(RvR + r1, w, r2);            // R2v in rows r1.. of the right block
+  pan.org2r(b, r2, vbuf, tau);
+  pan.store(Q2v, b, b, r2);
+  // ---- 3. U side: [U_ij, -U_ik W] = [Q1, Q2u] [[D, Cu], [0, R2u]]
+  for (int a = wv; a < r1; a += NW) {
+    double d = 0.0;
+    for (int p = ln; p < b; p += 64) { const double u = Uij[p + (int64_t)a * b]; d = fma(u, u, d); }
+    d = wave_sum(d);
+    if (ln == 0) { const double sgm = sqrt(d); sg[a] = sgm; sc[a] = sgm > 0.0 ? 1.0 / sgm : 0.0; }
   }
   __syncthreads();
-  // ---- U side: [U_ij, -U_ik W]
-#pragma unroll
-  for (int jj = 0; jj < 8; ++jj) {
-    const int c = wv + NW * jj;
-    if (c < r1) {
-#pragma unroll
-      for (int t = 0; t < RPL; ++t) {
-        const int p = ln + 64 * t;
-        pan.a[jj][t] = (p < b) ? Uij[p + (int64_t)c * b] : 0.0;
-      }
-    } else if (c < w) {
-      const int cc = c - r1;
-#pragma unroll
-      for (int t = 0; t < RPL; ++t) {
-        const int p = ln + 64 * t;
-        double s = 0.0;
-        if (p < b)
-          for (int a = 0; a < rik; ++a) s = fma(Uik[p + (int64_t)a * b], Wm[a + cc * rik], s);
-        pan.a[jj][t] = -s;
-      }
-    } else {
-#pragma unroll
-      for (int t = 0; t < RPL; ++t) pan.a[jj][t] = 0.0;
-    }
+  pan.zero();
+  panel_axpy(pan, Uik, b, b, rik, r2, (const double*)nullptr, Wm, rik, -1.0);
+  if (r1 > 0) {
+    panel_coef(pan, Uij, b, b, r1, r2, sc, RuR, w);
+    __syncthreads();
+    panel_axpy(pan, Uij, b, b, r1, r2, sc, RuR, w, -1.0);
+    panel_coef(pan, Uij, b, b, r1, r2, sc, Ct, r1);
+    __syncthreads();
+    panel_axpy(pan, Uij, b, b, r1, r2, sc, Ct, r1, -1.0);
+    for (int e = tid; e < r1 * r2; e += NT) RuR[(e % r1) + (e / r1) * w] += Ct[e];
   }
-  pan.geqr2(b, w, vbuf, tau);
-  pan.store_r(Ru, w, w);
+  pan.geqr2(b, r2, vbuf, tau);
+  pan.store_r(RuR + r1, w, r2);
   __syncthreads();
-  // ---- core G = R_u R_v^T
-  for (int e = tid; e < w * w; e += blockDim.x) {
+  // ---- 4. core G = R_u R_v^T, one-sided Jacobi, truncation
+  for (int e = tid; e < w * w; e += NT) {
     const int a = e % w, c = e / w;
-    double s = 0.0;
-    for (int m = max(a, c); m < w; ++m) s = fma(Ru[a + m * w], Rv[c + m * w], s);
+    double s = (a == c && a < r1) ? sg[a] : 0.0;
+    for (int m = 0; m < r2; ++m) s = fma(RuR[a + m * w], RvR[c + m * w], s);
     G[a + c * w] = s;
   }
   __syncthreads();
-  jacobi_svd<(NW >= 16 ? 16 : (NW >= 8 ? 8 : 4))>(G, w, J, w, w, sig, perm, &flag, 60);
+  const int sweeps = jacobi_svd<(NW >= 8 ? 8 : 4)>(G, w, J, w, w, sig, perm, &flag, 60);
+  if (sweeps == -2) {   // non-finite core: report, keep the tile unchanged
+    if (tid == 0) atomicMax(&P.status[ST_GUARD], 2);
+    return;
+  }
   if (tid == 0) {
     double tot = 0.0;
     for (int c = 0; c < w; ++c) tot += sig[c] * sig[c];
-    const double thr = rule_fixed ? thr_rel * sqrt(tot) : thr_abs;
-    // smallest r with sqrt(sum_{c >= r} s_c^2) <= thr
+    const double thr = sa.rule_fixed ? thr_rel * sqrt(tot) : thr_abs;
     double acc = 0.0;
     int r = w;
     for (int c = w - 1; c >= 0; --c) {
@@ -456,27 +513,47 @@ __global__ __launch_bounds__(64 * NW) void k_recompress(SlotArgs sa, int k, int 
   }
   __syncthreads();
   const int rn = s_rank;
-  // ---- V_ij = Q_v J[:, perm[:rn]]
-  for (int e = tid; e < b * rn; e += blockDim.x) {
-    const int p = e % b, c = e / b;
-    const int pc = perm[c];
-    double s = 0.0;
-    for (int m = 0; m < w; ++m) s = fma(Qs[p + (int64_t)m * b], J[m + pc * w], s);
-    Vij[p + (int64_t)c * b] = s;
+  // ---- 5. outputs, row blocks of 64 staged through LDS (in place)
+  pan.org2r(b, r2, vbuf, tau);   // Q2u in registers
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int r0 = 64 * t;
+    if (r0 >= b) break;
+    // V_ij <- [V_ij, Q2v] J[:, perm]
+    for (int e = tid; e < 64 * w; e += NT) {
+      const int rr = e % 64, m = e / 64, p = r0 + rr;
+      Sb[m * 64 + rr] = (p < b) ? (m < r1 ? Vij[p + (int64_t)m * b] : Q2v[p + (int64_t)(m - r1) * b]) : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < 64 * rn; e += NT) {
+      const int rr = e % 64, c = e / 64, p = r0 + rr;
+      const double* Jc = J + perm[c] * w;
+      double s = 0.0;
+      for (int m = 0; m < w; ++m) s = fma(Sb[m * 64 + rr], Jc[m], s);
+      if (p < b) Vij[p + (int64_t)c * b] = s;
+    }
+    __syncthreads();
+    // U_ij <- [Q1, Q2u] G[:, perm],  Q1 = U_ij diag(1/sigma)
+    for (int e = tid; e < 64 * r1; e += NT) {
+      const int rr = e % 64, m = e / 64, p = r0 + rr;
+      Sb[m * 64 + rr] = (p < b) ? Uij[p + (int64_t)m * b] * sc[m] : 0.0;
+    }
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+      const int c = RegPanel<RPL, NW>::col_of(wv, jj);
+      if (c < r2) Sb[(r1 + c) * 64 + ln] = pan.a[jj][t];
+    }
+    __syncthreads();
+    for (int e = tid; e < 64 * rn; e += NT) {
+      const int rr = e % 64, c = e / 64, p = r0 + rr;
+      const double* Gc = G + perm[c] * w;
+      double s = 0.0;
+      for (int m = 0; m < w; ++m) s = fma(Sb[m * 64 + rr], Gc[m], s);
+      if (p < b) Uij[p + (int64_t)c * b] = s;
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  // ---- U_ij = Q_u G[:, perm[:rn]]
-  pan.org2r(b, w, vbuf, tau);
-  pan.store(Qs, b, b, w);
-  __syncthreads();
-  for (int e = tid; e < b * rn; e += blockDim.x) {
-    const int p = e % b, c = e / b;
-    const int pc = perm[c];
-    double s = 0.0;
-    for (int m = 0; m < w; ++m) s = fma(Qs[p + (int64_t)m * b], G[m + pc * w], s);
-    Uij[p + (int64_t)c * b] = s;
-  }
-  if (tid == 0) { P.rank[tij] = rn; P.done_k[tij] = k; }
+  if (tid == 0) P.rank[tij] = rn;
 }
 
 // ------------------------------------------------------------------------------------ final
@@ -525,40 +602,34 @@ hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t 
   return hipGetLastError();
 }
 
-template <int RPL, int NW, bool BIG>
-static hipError_t launch_recomp_t(const SlotArgs& sa, int k, int npairs, int nslot, int wlo, int whi, int rule_fixed,
-                                  double thr_rel, double thr_abs, hipStream_t s) {
-  constexpr int WMAX = 8 * NW;
-  const size_t lds = BIG ? (size_t)WMAX * WMAX * sizeof(double) : (size_t)4 * WMAX * WMAX * sizeof(double);
+template <int RPL, int NW>
+static hipError_t launch_recomp_t(const SlotArgs& sa, int k, int npairs, int nslot, int r2lo, int r2hi, hipStream_t s) {
+  const size_t lds = (size_t)RecompLayout<NW>::TOTAL * sizeof(double);
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void*)k_recompress<RPL, NW, BIG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)lds);
+    (void)hipFuncSetAttribute((const void*)k_recompress<RPL, NW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_recompress<RPL, NW, BIG>), dim3(npairs, 1, nslot), dim3(64 * NW), lds, s, sa, k, wlo, whi,
-                     rule_fixed, thr_rel, thr_abs);
+  hipLaunchKernelGGL((k_recompress<RPL, NW>), dim3(npairs, 1, nslot), dim3(64 * NW), lds, s, sa, k, r2lo, r2hi);
   return hipGetLastError();
 }
 
-hipError_t launch_recompress(const SlotArgs& sa, int k, int nslot, int rule_fixed, double thr_rel, double thr_abs,
-                             hipStream_t s) {
+hipError_t launch_recompress(const SlotArgs& sa, int k, int nslot, hipStream_t s) {
   const int m = sa.nB - k - 1;            // tiles below the panel
   const int npairs = m * (m - 1) / 2;     // pairs (i, j), k < j < i
   if (npairs <= 0) return hipSuccess;
+  // r_jk <= 32: 4 waves; up to 64: 8 waves (the class depends only on the column-k rank, which this
+  // step does not change, so each pair is handled by exactly one launch)
   hipError_t e;
   if (sa.b <= 256) {
-    e = launch_recomp_t<4, 4, false>(sa, k, npairs, nslot, 1, 32, rule_fixed, thr_rel, thr_abs, s);
+    e = launch_recomp_t<4, 4>(sa, k, npairs, nslot, 1, 32, s);
     if (e != hipSuccess) return e;
-    e = launch_recomp_t<4, 8, false>(sa, k, npairs, nslot, 33, 64, rule_fixed, thr_rel, thr_abs, s);
-    if (e != hipSuccess) return e;
-    e = launch_recomp_t<4, 16, true>(sa, k, npairs, nslot, 65, 128, rule_fixed, thr_rel, thr_abs, s);
+    e = launch_recomp_t<4, 8>(sa, k, npairs, nslot, 33, 64, s);
   } else {
-    e = launch_recomp_t<8, 4, false>(sa, k, npairs, nslot, 1, 32, rule_fixed, thr_rel, thr_abs, s);
+    e = launch_recomp_t<8, 4>(sa, k, npairs, nslot, 1, 32, s);
     if (e != hipSuccess) return e;
-    e = launch_recomp_t<8, 8, false>(sa, k, npairs, nslot, 33, 64, rule_fixed, thr_rel, thr_abs, s);
-    if (e != hipSuccess) return e;
-    e = launch_recomp_t<8, 16, true>(sa, k, npairs, nslot, 65, 128, rule_fixed, thr_rel, thr_abs, s);
+    e = launch_recomp_t<8, 8>(sa, k, npairs, nslot, 33, 64, s);
   }
   return e;
 }

@@ -2,6 +2,7 @@
 //  * RegPanel: an m x n panel (m <= 64*RPL, n <= 8*NW) distributed over NW waves; wave w owns the
 //    columns j = w + NW*jj (jj < 8), lane l owns the rows p = l + 64*t (t < RPL).  Householder QR with
 //    LAPACK dlarfg/dgeqr2 conventions, explicit Q in place (dorg2r), all without spilling the panel.
+//  * panel_coef / panel_axpy: projections of the panel against a global matrix (block Gram-Schmidt).
 //  * jacobi_svd: one-sided (Hestenes) Jacobi on the columns of a small square matrix, round-robin
 //    ordering, rotations accumulated into J.
 #pragma once
@@ -187,11 +188,64 @@ struct RegPanel {
   }
 };
 
+// C[a + j*ldc] = sc[a] * (A[:, a]^T P[:, j]) for a < na, j < n; A global (m x na, column-major, lda),
+// sc optional column scale.  Each wave produces the entries of its own panel columns.
+template <int RPL, int NW>
+__device__ __forceinline__ void panel_coef(const RegPanel<RPL, NW>& P, const double* A, int64_t lda, int m, int na,
+                                           int n, const double* sc, double* C, int ldc) {
+  const int w = wave_id(), l = lane_id();
+  for (int a = 0; a < na; ++a) {
+    double av[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int p = l + 64 * t;
+      av[t] = (p < m) ? A[p + (int64_t)a * lda] : 0.0;
+    }
+    const double s = sc ? sc[a] : 1.0;
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+      const int j = RegPanel<RPL, NW>::col_of(w, jj);
+      if (j < n) {
+        double d = 0.0;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) d += av[t] * P.a[jj][t];
+        d = wave_sum(d);
+        if (l == 0) C[a + j * ldc] = d * s;
+      }
+    }
+  }
+}
+
+// P[:, j] += alpha * sum_a A[:, a] sc[a] C[a + j*ldc] for j < n (C must be visible to every wave).
+template <int RPL, int NW>
+__device__ __forceinline__ void panel_axpy(RegPanel<RPL, NW>& P, const double* A, int64_t lda, int m, int na, int n,
+                                           const double* sc, const double* C, int ldc, double alpha) {
+  const int w = wave_id(), l = lane_id();
+  for (int a = 0; a < na; ++a) {
+    double av[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int p = l + 64 * t;
+      av[t] = (p < m) ? A[p + (int64_t)a * lda] : 0.0;
+    }
+    const double s = alpha * (sc ? sc[a] : 1.0);
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+      const int j = RegPanel<RPL, NW>::col_of(w, jj);
+      if (j < n) {
+        const double f = s * C[a + j * ldc];
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) P.a[jj][t] = fma(f, av[t], P.a[jj][t]);
+      }
+    }
+  }
+}
+
 // One-sided Jacobi SVD of the n x n matrix G (column-major, ldg; n <= 2*(NT/TP)), rotations
 // accumulated into J (initialised to I here).  On return the columns of G are mutually orthogonal,
 // G_out = G_in J, sig[j] = ||G_out[:, j]|| and perm[0..n) lists columns by descending sig (ties by
 // index).  TP threads cooperate on one column pair.  Returns the number of sweeps (-1: no convergence
-// within max_sweeps).  G, J, sig, perm may live in LDS or global memory; flag is an LDS int.
+// within max_sweeps, -2: a non-finite column norm).  G, J, sig, perm may live in LDS or global memory; flag is an LDS int.
 template <int TP>
 __device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double* sig, int* perm, int* flag,
                           int max_sweeps) {
@@ -248,7 +302,10 @@ __device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double*
   for (int j = tid; j < n; j += nt) {
     double acc = 0.0;
     for (int i = 0; i < n; ++i) { const double g = G[i + j * ldg]; acc += g * g; }
-    sig[j] = sqrt(acc);
+    const double sv = sqrt(acc);
+    // a non-finite column norm is ranked last (as -1) so perm stays a permutation; the caller sees it
+    // through the returned status
+    sig[j] = (sv == sv && sv <= 1.7976931348623157e308) ? sv : -1.0;
   }
   __syncthreads();
   for (int j = tid; j < n; j += nt) {
@@ -261,6 +318,8 @@ __device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double*
     perm[rank] = j;
   }
   __syncthreads();
+  for (int j = 0; j < n; ++j)
+    if (sig[j] < 0.0) return -2;
   return sweeps;
 }
 

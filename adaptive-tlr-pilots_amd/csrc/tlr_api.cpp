@@ -10,6 +10,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -69,6 +70,8 @@ struct tlr_ctx {
   SlotPtrs* d_slots = nullptr;
   int64_t scr_stride = 0;
   int* d_retry = nullptr;              // per slot: tiles redone with the wider sketch
+  double* d_thr = nullptr;             // per slot (thr_rel, thr_abs) of the current batch
+  double* h_thr = nullptr;             // pinned staging of d_thr
   // timing mode: launches of kernel class timing_kernel are bracketed by events
   int timing_kernel = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -77,6 +80,7 @@ struct tlr_ctx {
   double timed_ms = 0.0, timed_flops = 0.0;
   int64_t timed_launches = 0;
   ~tlr_ctx() {
+    if (h_thr) hipHostFree(h_thr);
     for (hipEvent_t e : ev_pool) hipEventDestroy(e);
     for (void* p : allocs) hipFree(p);
     if (stream) hipStreamDestroy(stream);
@@ -112,6 +116,8 @@ struct tlr_ctx {
     a.n = n; a.b = b; a.nB = nB; a.npan = npan; a.rank_cap = RANK_CAP;
     a.fstride = (int64_t)b * RANK_CAP;
     a.scr_stride = scr_stride;
+    a.rule_fixed = 0;
+    a.thr = d_thr;
     a.slots = d_slots;
     return a;
   }
@@ -194,7 +200,9 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
   ACHK(c->alloc(&c->bufR, (size_t)nt * c->kcap * c->kcap));
   ACHK(c->alloc(&c->part, (size_t)nt * nbb * nbb * 2));
   ACHK(c->alloc(&c->need, (size_t)nt));
-  c->scr_stride = (int64_t)b * WCAP + 3 * WCAP * WCAP;
+  // recompression scratch per pair: Q2v (b x 64), the right blocks of R_u, R_v (128 x 64 each) and the
+  // Jacobi core and rotations (128 x 128 each) when they do not fit in LDS
+  c->scr_stride = (int64_t)b * 64 + 2 * WCAP * 64 + 2 * WCAP * WCAP;
   const int64_t npm = c->npairs_max > 0 ? c->npairs_max : 1;
   c->slots.resize(max_batch);
   for (int s = 0; s < max_batch; ++s) {
@@ -205,7 +213,6 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
     ACHK(c->alloc(&p.V, (size_t)nt * b * RANK_CAP));
     ACHK(c->alloc(&p.rank, (size_t)nt));
     ACHK(c->alloc(&p.rank0, (size_t)nt));
-    ACHK(c->alloc(&p.done_k, (size_t)nt));
     ACHK(c->alloc(&p.Wd, (size_t)c->nB * RANK_CAP * RANK_CAP));
     ACHK(c->alloc(&p.Xd, (size_t)c->nB * b * RANK_CAP));
     ACHK(c->alloc(&p.rhs, (size_t)n));
@@ -216,6 +223,8 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
   }
   ACHK(c->alloc(&c->d_slots, (size_t)max_batch));
   ACHK(c->alloc(&c->d_retry, (size_t)max_batch));
+  ACHK(c->alloc(&c->d_thr, (size_t)2 * max_batch));
+  ACHK(hipHostMalloc((void**)&c->h_thr, sizeof(double) * 2 * max_batch, hipHostMallocDefault));
   std::vector<SlotPtrs> hp(max_batch);
   for (int s = 0; s < max_batch; ++s) hp[s] = c->slots[s].p;
   ACHK(hipMemcpy(c->d_slots, hp.data(), max_batch * sizeof(SlotPtrs), hipMemcpyHostToDevice));
@@ -318,6 +327,15 @@ int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
   c->ev_used = 0;
   c->timed.clear();
   HIPCHK(launch_fill_int(c->d_retry, c->max_batch, 0, st));
+  // per-slot thresholds of the recompression (the adaptive budget depends on theta); the previous call
+  // ended with a synchronisation, so the pinned staging buffer is free
+  const double bmax_ = (double)(c->n < c->b ? c->n : c->b);
+  for (int s = 0; s < m; ++s) {
+    const double* th = thetas + 3 * s;
+    c->h_thr[2 * s] = rule == TLR_RULE_FIXED ? value : 0.0;
+    c->h_thr[2 * s + 1] = rule == TLR_RULE_FIXED ? 0.0 : value * bmax_ * th[0] * th[2] / (double)c->n;
+  }
+  HIPCHK(hipMemcpyAsync(c->d_thr, c->h_thr, sizeof(double) * 2 * m, hipMemcpyHostToDevice, st));
   const double bmax = (double)(c->n < c->b ? c->n : c->b);
   for (int s = 0; s < m; ++s) {
     const double* th = thetas + 3 * s;
@@ -329,29 +347,17 @@ int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
     const double thr_abs = rule_fixed ? 0.0 : value * bmax * th[0] * th[2] / (double)c->n;
     HIPCHK(launch_fill_int(p.status, ST_COUNT, 0, st));
     HIPCHK(launch_fill_int(p.status + ST_FAILED, 1, 0x7fffffff, st));
-    HIPCHK(launch_fill_int(p.done_k, c->ntile, -1, st));
     HIPCHK(hipMemcpyAsync(p.rhs, c->y, c->n * sizeof(double), hipMemcpyDeviceToDevice, st));
     int rc = compress_slot(c, s, p, mp, rule_fixed, value, thr_abs);
     if (rc != TLR_OK) return rc;
   }
-  const SlotArgs sa = c->args();
-  // recompression thresholds: all slots of one batch share the rule, but thr_abs depends on theta;
-  // launches take one threshold, so batches with the adaptive rule run the recompression per slot
+  SlotArgs sa = c->args();
+  sa.rule_fixed = rule_fixed;
   for (int k = 0; k < c->nB; ++k) {
     HIPCHK(launch_potrf(sa, k, m, st));
     HIPCHK(launch_trsm(sa, k, m, st));
     HIPCHK(launch_diag_update(sa, k, m, st));
-    if (rule_fixed) {
-      HIPCHK(launch_recompress(sa, k, m, 1, value, 0.0, st));
-    } else {
-      for (int s = 0; s < m; ++s) {
-        const double* th = thetas + 3 * s;
-        const double thr_abs = value * bmax * th[0] * th[2] / (double)c->n;
-        SlotArgs one = sa;
-        one.slots = c->d_slots + s;
-        HIPCHK(launch_recompress(one, k, 1, 0, 0.0, thr_abs, st));
-      }
-    }
+    HIPCHK(launch_recompress(sa, k, m, st));
   }
   HIPCHK(launch_final(sa, m, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -377,6 +383,7 @@ int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
     HIPCHK(hipMemcpy(stv, p.status, sizeof(stv), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(res, p.result, sizeof(res), hipMemcpyDeviceToHost));
     if (stv[ST_RANKCAP]) return set_err(TLR_ERR_RANKCAP, "tile rank exceeded the capacity of 64 columns");
+    if (stv[ST_GUARD]) return set_err(TLR_ERR_HIP, "a kernel guard tripped (rank or core out of range)");
     const int fa = stv[ST_FAILED];
     const bool ok = fa == 0x7fffffff;
     out[3 * s + 0] = ok ? res[0] : NAN;

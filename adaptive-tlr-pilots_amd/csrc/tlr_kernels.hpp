@@ -6,7 +6,9 @@
 
 namespace tlr {
 
-enum { ST_RANKCAP = 0, ST_FAILED = 1, ST_COUNT = 4 };
+// status words: rank capacity exceeded; first non-PD tile (INT_MAX: none); a kernel guard tripped
+// (rank or core out of range: the result is unusable and reported as an error)
+enum { ST_RANKCAP = 0, ST_FAILED = 1, ST_GUARD = 2, ST_COUNT = 4 };
 
 // Device buffers of one slot (one theta being evaluated).  Layouts (all FP64 column-major):
 //   D      nB tiles of b x b: diagonal tiles, factored in place into L_kk (lower)
@@ -23,7 +25,6 @@ struct SlotPtrs {
   double* V;
   int* rank;
   int* rank0;
-  int* done_k;
   double* Wd;
   double* Xd;
   double* rhs;
@@ -38,6 +39,8 @@ struct SlotArgs {
   int b, nB, npan, rank_cap;
   int64_t fstride;      // b * rank_cap
   int64_t scr_stride;   // doubles of scratch per recompression pair
+  int rule_fixed;       // 1: fixed relative tolerance, 0: adaptive absolute budget
+  const double* thr;    // per slot: (thr_rel, thr_abs)
   const SlotPtrs* slots;
   __device__ __forceinline__ SlotPtrs slot(int s) const { return slots[s]; }
 };
@@ -65,8 +68,7 @@ hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t 
 hipError_t launch_potrf(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_trsm(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t s);
-hipError_t launch_recompress(const SlotArgs& sa, int k, int nslot, int rule_fixed, double thr_rel, double thr_abs,
-                             hipStream_t s);
+hipError_t launch_recompress(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_final(const SlotArgs& sa, int nslot, hipStream_t s);
 
 }  // namespace tlr

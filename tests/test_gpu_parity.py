@@ -126,3 +126,21 @@ def test_n40000_b250_golden(tlrgp, goldens):
     assert info["stored_factor"] == r["stored_factor"]
     assert info["max_rank_factor"] == r["max_rank_factor"]
     assert _close(ll, r["loglik"]), (ll, r["loglik"])
+
+
+def test_n40000_b500_vs_oracle(tlrgp):
+    """The north-star configuration (N=40,000, b=500, adaptive 0.9) at theta=(1, 0.3, 0.05) on the seed-0
+    field: every initial and final tile rank and log L / log-det / quadratic form against the oracle's
+    stored result (tests/golden/make_seed0_oracle.py)."""
+    path = os.path.join(GOLDEN, "seed0_n40000_b500_oracle.npz")
+    if not os.path.exists(path):
+        pytest.skip("oracle fixture not generated")
+    ref = np.load(path)
+    S, y = _seed0_field(40000)
+    print("case N=40000 b=500 adaptive:0.9 seed0", flush=True)
+    ll, fac, info = tlrgp.tlr_loglik(S, y, ref["theta"], tlrgp.tile_bounds(40000, 500), "adaptive", 0.9)
+    np.testing.assert_array_equal(info["ranks_sigma"], ref["ranks_sigma"])
+    np.testing.assert_array_equal(info["ranks_factor"], ref["ranks_factor"])
+    assert _close(ll, float(ref["loglik"])), (ll, float(ref["loglik"]))
+    assert _close(info["logdet"], float(ref["logdet"]))
+    assert _close(info["quad"], float(ref["quad"]))
