@@ -239,7 +239,8 @@ __global__ __launch_bounds__(256) void k_rank_decide(const double* __restrict__ 
                                                      int kmax, int pmin, int rule_fixed, double thr_rel, double thr_abs,
                                                      double* __restrict__ U, double* __restrict__ V, int64_t fstride,
                                                      int rank_cap, int* __restrict__ rank, int* __restrict__ need,
-                                                     int* __restrict__ retry_count, int* __restrict__ status) {
+                                                     int* __restrict__ retry_count, int* __restrict__ status,
+                                                     const int* __restrict__ forced) {
   const int64_t t = blockIdx.x;
   if (!need[t]) return;
   extern __shared__ double lds[];
@@ -279,8 +280,13 @@ __global__ __launch_bounds__(256) void k_rank_decide(const double* __restrict__ 
     }
     const double thr = rule_fixed ? thr_rel * sqrt(s_fro2) : thr_abs;
     int r = -1;
-    for (int c = 0; c <= k; ++c)
-      if (tail[c] <= thr) { r = c; break; }
+    if (forced) {
+      // replay of a recorded evaluation: the rank is given, the sketch must still leave pmin spare columns
+      r = forced[t] <= k ? forced[t] : -1;
+    } else {
+      for (int c = 0; c <= k; ++c)
+        if (tail[c] <= thr) { r = c; break; }
+    }
     int accept;
     if (r < 0) accept = 0;                              // residual alone exceeds the budget
     else accept = (r + pmin <= k) || (k >= kmax);
@@ -397,7 +403,8 @@ size_t rank_decide_lds(int k) {
 hipError_t launch_rank_decide(const double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
                               int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
                               double thr_rel, double thr_abs, double* U, double* V, int64_t fstride, int rank_cap,
-                              int* rank, int* need, int* retry_count, int* status, int64_t ntile, hipStream_t s) {
+                              int* rank, int* need, int* retry_count, int* status, const int* forced, int64_t ntile,
+                              hipStream_t s) {
   const size_t lds = rank_decide_lds(k);
   if (lds > 158 * 1024) return hipErrorInvalidValue;
   static bool attr_set = false;
@@ -407,7 +414,7 @@ hipError_t launch_rank_decide(const double* Rm, int64_t rstride, const double* Q
   }
   hipLaunchKernelGGL(k_rank_decide, dim3((unsigned)ntile), dim3(256), lds, s, Rm, rstride, Qrow, Qcol, qstride, part,
                      b, k, kmax, pmin, rule_fixed, thr_rel, thr_abs, U, V, fstride, rank_cap, rank, need, retry_count,
-                     status);
+                     status, forced);
   return hipGetLastError();
 }
 

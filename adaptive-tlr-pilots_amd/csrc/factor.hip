@@ -508,7 +508,11 @@ __global__ __launch_bounds__(64 * NW) void k_recompress(SlotArgs sa, int k, int 
       const double nacc = acc + s * s;
       if (sqrt(nacc) <= thr) { acc = nacc; r = c; } else break;
     }
+    const int64_t lo = sa.rlog_off[k] + blockIdx.x;
+    const int rep = sa.replay[slot];
+    if (rep >= 0) r = min(sa.rep_rlog[(int64_t)rep * sa.nlog + lo], w);   // frozen ranks
     if (r > sa.rank_cap) { atomicMax(&P.status[ST_RANKCAP], 1); r = sa.rank_cap; }
+    P.rlog[lo] = r;
     s_rank = r;
   }
   __syncthreads();

@@ -72,6 +72,12 @@ struct tlr_ctx {
   int* d_retry = nullptr;              // per slot: tiles redone with the wider sketch
   double* d_thr = nullptr;             // per slot (thr_rel, thr_abs) of the current batch
   double* h_thr = nullptr;             // pinned staging of d_thr
+  int* d_replay = nullptr;             // per slot: record whose ranks are replayed, or -1
+  int* h_replay = nullptr;             // pinned staging of d_replay
+  int64_t nlog = 0;                    // recompressions per evaluation (rank-log length)
+  int64_t* d_rlog_off = nullptr;       // nB offsets into a rank log
+  int* rep_rank0 = nullptr;            // max_batch records of initial ranks (ntile each)
+  int* rep_rlog = nullptr;             // max_batch records of recompression ranks (nlog each)
   // timing mode: launches of kernel class timing_kernel are bracketed by events
   int timing_kernel = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -81,6 +87,7 @@ struct tlr_ctx {
   int64_t timed_launches = 0;
   ~tlr_ctx() {
     if (h_thr) hipHostFree(h_thr);
+    if (h_replay) hipHostFree(h_replay);
     for (hipEvent_t e : ev_pool) hipEventDestroy(e);
     for (void* p : allocs) hipFree(p);
     if (stream) hipStreamDestroy(stream);
@@ -118,6 +125,10 @@ struct tlr_ctx {
     a.scr_stride = scr_stride;
     a.rule_fixed = 0;
     a.thr = d_thr;
+    a.replay = d_replay;
+    a.rep_rlog = rep_rlog;
+    a.rlog_off = d_rlog_off;
+    a.nlog = nlog;
     a.slots = d_slots;
     return a;
   }
@@ -221,6 +232,24 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
     ACHK(c->alloc(&p.result, (size_t)4));
     ACHK(c->alloc(&p.scratch, (size_t)npm * c->scr_stride));
   }
+  // rank logs: offsets of each step's recompressions, one log per slot and max_batch replay records
+  {
+    std::vector<int64_t> off(c->nB);
+    int64_t acc = 0;
+    for (int k = 0; k < c->nB; ++k) {
+      off[k] = acc;
+      const int64_t mk = c->nB - k - 1;
+      acc += mk * (mk - 1) / 2;
+    }
+    c->nlog = acc > 0 ? acc : 1;
+    ACHK(c->alloc(&c->d_rlog_off, (size_t)c->nB));
+    ACHK(hipMemcpy(c->d_rlog_off, off.data(), c->nB * sizeof(int64_t), hipMemcpyHostToDevice));
+  }
+  for (int s = 0; s < max_batch; ++s) ACHK(c->alloc(&c->slots[s].p.rlog, (size_t)c->nlog));
+  ACHK(c->alloc(&c->rep_rank0, (size_t)max_batch * nt));
+  ACHK(c->alloc(&c->rep_rlog, (size_t)max_batch * c->nlog));
+  ACHK(c->alloc(&c->d_replay, (size_t)max_batch));
+  ACHK(hipHostMalloc((void**)&c->h_replay, sizeof(int) * max_batch, hipHostMallocDefault));
   ACHK(c->alloc(&c->d_slots, (size_t)max_batch));
   ACHK(c->alloc(&c->d_retry, (size_t)max_batch));
   ACHK(c->alloc(&c->d_thr, (size_t)2 * max_batch));
@@ -253,7 +282,7 @@ int tlr_ctx_set_y(tlr_ctx* ctx, const double* y) {
 }
 
 static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternParams& mp, int rule_fixed,
-                         double thr_rel, double thr_abs) {
+                         double thr_rel, double thr_abs, const int* forced) {
   hipStream_t s = c->stream;
   const int b = c->b;
   const int64_t nt = c->ntile;
@@ -306,14 +335,18 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
     const double* Qcol = (last_op == 0) ? Qprev : M;
     HIPCHK(launch_rank_decide(c->bufR, rs, Qrow, Qcol, ms, c->part, b, k, kmax, PMIN, rule_fixed, thr_rel, thr_abs,
                               p.U, p.V, (int64_t)b * RANK_CAP, RANK_CAP, p.rank, c->need, c->d_retry + slot,
-                              p.status, nt, s));
+                              p.status, forced, nt, s));
   }
   HIPCHK(launch_copy_int(p.rank0, p.rank, nt, s));
   return TLR_OK;
 }
 
-int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
-                     double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma, int32_t* ranks_factor) {
+}  // extern "C"
+
+// One pass over m theta; replay[s] >= 0 makes slot s reuse the ranks of record replay[s] (frozen ranks).
+static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
+                     const int* replay, double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma,
+                     int32_t* ranks_factor) {
   if (!c || !thetas || !out || m < 1 || m > c->max_batch) return set_err(TLR_ERR_ARG, "invalid argument");
   if (rule != TLR_RULE_FIXED && rule != TLR_RULE_ADAPTIVE) return set_err(TLR_ERR_ARG, "unknown rule");
   if (precision != TLR_PREC_FP64) return set_err(TLR_ERR_ARG, "unsupported precision");
@@ -336,6 +369,8 @@ int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
     c->h_thr[2 * s + 1] = rule == TLR_RULE_FIXED ? 0.0 : value * bmax_ * th[0] * th[2] / (double)c->n;
   }
   HIPCHK(hipMemcpyAsync(c->d_thr, c->h_thr, sizeof(double) * 2 * m, hipMemcpyHostToDevice, st));
+  for (int s = 0; s < m; ++s) c->h_replay[s] = replay ? replay[s] : -1;
+  HIPCHK(hipMemcpyAsync(c->d_replay, c->h_replay, sizeof(int) * m, hipMemcpyHostToDevice, st));
   const double bmax = (double)(c->n < c->b ? c->n : c->b);
   for (int s = 0; s < m; ++s) {
     const double* th = thetas + 3 * s;
@@ -348,7 +383,8 @@ int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
     HIPCHK(launch_fill_int(p.status, ST_COUNT, 0, st));
     HIPCHK(launch_fill_int(p.status + ST_FAILED, 1, 0x7fffffff, st));
     HIPCHK(hipMemcpyAsync(p.rhs, c->y, c->n * sizeof(double), hipMemcpyDeviceToDevice, st));
-    int rc = compress_slot(c, s, p, mp, rule_fixed, value, thr_abs);
+    const int* forced = (replay && replay[s] >= 0) ? c->rep_rank0 + (int64_t)replay[s] * c->ntile : nullptr;
+    int rc = compress_slot(c, s, p, mp, rule_fixed, value, thr_abs, forced);
     if (rc != TLR_OK) return rc;
   }
   SlotArgs sa = c->args();
@@ -423,6 +459,77 @@ int tlr_ctx_get_timing(const tlr_ctx* ctx, double* total_ms, int64_t* launches, 
   if (launches) *launches = ctx->timed_launches;
   if (flops) *flops = ctx->timed_flops;
   return TLR_OK;
+}
+
+extern "C" {
+
+int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
+                     double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma, int32_t* ranks_factor) {
+  return run_batch(c, m, thetas, rule, value, precision, nullptr, out, status, failed_at, ranks_sigma, ranks_factor);
+}
+
+int tlr_loglik_grad_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
+                          double h, double* out, double* grad, int32_t* status, int32_t* failed_at,
+                          int32_t* ranks_sigma, int32_t* ranks_factor) {
+  if (!c || !grad || !(h > 0.0) || h > 0.1) return set_err(TLR_ERR_ARG, "invalid argument");
+  std::vector<int32_t> st(m > 0 ? m : 1);
+  int rc = run_batch(c, m, thetas, rule, value, precision, nullptr, out, st.data(), failed_at, ranks_sigma,
+                     ranks_factor);
+  if (rc < 0) return rc;
+  if (status) memcpy(status, st.data(), sizeof(int32_t) * m);
+  // keep the base ranks as replay records
+  for (int s = 0; s < m; ++s) {
+    const SlotPtrs& p = c->slots[s].p;
+    if (c->ntile > 0)
+      HIPCHK(hipMemcpyAsync(c->rep_rank0 + (int64_t)s * c->ntile, p.rank0, c->ntile * sizeof(int),
+                            hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->rep_rlog + (int64_t)s * c->nlog, p.rlog, c->nlog * sizeof(int), hipMemcpyDeviceToDevice,
+                          c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // d/dlog sigma2 in closed form: tiles and budgets both scale with sigma2, so the ranks are unchanged, L
+  // scales by sigma and log L = -(quad / sigma2 + logdet + N log sigma2 + N log 2 pi) / 2 in sigma2
+  // d/dlog range, d/dlog eta: central differences at the base ranks
+  std::vector<double> th4;
+  std::vector<int> src;
+  for (int s = 0; s < m; ++s) {
+    grad[3 * s + 0] = st[s] == TLR_OK ? 0.5 * out[3 * s + 2] - 0.5 * (double)c->n : NAN;
+    grad[3 * s + 1] = grad[3 * s + 2] = NAN;
+    if (st[s] != TLR_OK) continue;
+    for (int par = 1; par <= 2; ++par)
+      for (int sg = -1; sg <= 1; sg += 2) {
+        const double* t = thetas + 3 * s;
+        th4.insert(th4.end(), {t[0], t[1], t[2]});
+        th4[th4.size() - 3 + par] *= exp(sg * h);
+        src.push_back(s);
+      }
+  }
+  const int n4 = (int)src.size();
+  std::vector<double> o4(3 * (n4 > 0 ? n4 : 1));
+  std::vector<int32_t> s4(n4 > 0 ? n4 : 1), f4(n4 > 0 ? n4 : 1);
+  for (int off = 0; off < n4; off += c->max_batch) {
+    const int mm = n4 - off < c->max_batch ? n4 - off : c->max_batch;
+    rc = run_batch(c, mm, th4.data() + 3 * off, rule, value, precision, src.data() + off, o4.data() + 3 * off,
+                   s4.data() + off, f4.data() + off, nullptr, nullptr);
+    if (rc < 0) return rc;
+  }
+  for (int e = 0; e < n4; e += 4) {
+    const int s = src[e];
+    for (int par = 1; par <= 2; ++par) {
+      const int ip = e + 2 * (par - 1) + 1, im = e + 2 * (par - 1);
+      if (s4[ip] == TLR_OK && s4[im] == TLR_OK) grad[3 * s + par] = (o4[3 * ip] - o4[3 * im]) / (2.0 * h);
+    }
+  }
+  return (m == 1 && st[0] != TLR_OK) ? TLR_NOT_PD : TLR_OK;
+}
+
+int tlr_loglik_grad(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, int32_t precision, double h,
+                    double out3[3], double grad3[3], int32_t* ranks_sigma, int32_t* ranks_factor, int32_t* failed_at) {
+  int32_t st = 0, fa = -1;
+  const int rc = tlr_loglik_grad_batch(ctx, 1, theta, rule, value, precision, h, out3, grad3, &st, &fa, ranks_sigma,
+                                       ranks_factor);
+  if (failed_at) *failed_at = fa;
+  return rc;
 }
 
 int tlr_loglik(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, int32_t precision, double out3[3],

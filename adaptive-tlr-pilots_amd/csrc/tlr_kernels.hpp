@@ -32,6 +32,7 @@ struct SlotPtrs {
   int* status;
   double* result;   // [loglik, logdet, quad]
   double* scratch;
+  int* rlog;          // rank chosen by every recompression, at rlog_off[k] + pair index
 };
 
 struct SlotArgs {
@@ -41,6 +42,10 @@ struct SlotArgs {
   int64_t scr_stride;   // doubles of scratch per recompression pair
   int rule_fixed;       // 1: fixed relative tolerance, 0: adaptive absolute budget
   const double* thr;    // per slot: (thr_rel, thr_abs)
+  const int* replay;    // per slot: -1, or the record whose ranks this evaluation reuses (frozen ranks)
+  const int* rep_rlog;  // records: recompression ranks, nlog per record
+  const int64_t* rlog_off;   // nB offsets of step k in a rank log
+  int64_t nlog;
   const SlotPtrs* slots;
   __device__ __forceinline__ SlotPtrs slot(int s) const { return slots[s]; }
 };
@@ -60,7 +65,8 @@ size_t rank_decide_lds(int k);
 hipError_t launch_rank_decide(const double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
                               int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
                               double thr_rel, double thr_abs, double* U, double* V, int64_t fstride, int rank_cap,
-                              int* rank, int* need, int* retry_count, int* status, int64_t ntile, hipStream_t s);
+                              int* rank, int* need, int* retry_count, int* status, const int* forced, int64_t ntile,
+                              hipStream_t s);
 hipError_t launch_fill_int(int* p, int64_t count, int value, hipStream_t s);
 hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t s);
 

@@ -151,6 +151,25 @@ class TLRContext:
         return dict(loglik=out[:, 0], logdet=out[:, 1], quad=out[:, 2], completed=st == _lib.TLR_OK,
                     failed_at=fa, ranks_sigma=rs, ranks_factor=rf)
 
+    def loglik_grad_batch(self, thetas, rule, value, h=1e-5, precision="fp64", ranks=True):
+        """As loglik_batch plus grad (m, 3): d log L / d(log sigma2, log range, log eta) at frozen ranks (the
+        sigma2 component exact, the others central differences with step h replaying the recorded ranks).
+        Needs max_batch >= 1; the 4 m perturbed evaluations run in chunks of max_batch."""
+        th = f64(np.atleast_2d(thetas))
+        m = th.shape[0]
+        if th.shape[1] != 3 or m > self.max_batch:
+            raise ValueError(f"thetas must be (m, 3) with m <= {self.max_batch}")
+        out = np.empty((m, 3))
+        grad = np.empty((m, 3))
+        st = np.empty(m, np.int32)
+        fa = np.empty(m, np.int32)
+        rs = np.zeros((m, self.nB, self.nB), np.int32) if ranks else None
+        rf = np.zeros((m, self.nB, self.nB), np.int32) if ranks else None
+        check(lib.tlr_loglik_grad_batch(self._ptr, m, dptr(th), RULES[rule], float(value), PRECISIONS[precision],
+                                        float(h), dptr(out), dptr(grad), iptr(st), iptr(fa), iptr(rs), iptr(rf)))
+        return dict(loglik=out[:, 0], logdet=out[:, 1], quad=out[:, 2], grad=grad, completed=st == _lib.TLR_OK,
+                    failed_at=fa, ranks_sigma=rs, ranks_factor=rf)
+
     def loglik(self, theta, rule, value, precision="fp64"):
         """One theta: (ll | None, TLRFactor | None, info) as tlr_loglik of 17:259-266."""
         r = self.loglik_batch(np.asarray(theta, float)[None, :], rule, value, precision)
@@ -190,6 +209,22 @@ def tlr_loglik(S, y, theta, bounds, rule, value, precision="fp64"):
     S = f64(S)
     b = _tile_size_from_bounds(S.shape[0], bounds)
     return _context_for(S, y, b).loglik(theta, rule, value, precision)
+
+
+def tlr_loglik_grad(S, y, theta, bounds, rule, value, h=1e-5, precision="fp64"):
+    """(ll | None, grad in (log sigma2, log range, log eta) | None, info) at frozen ranks."""
+    S = f64(S)
+    b = _tile_size_from_bounds(S.shape[0], bounds)
+    ctx = _context_for(S, y, b)
+    r = ctx.loglik_grad_batch(np.asarray(theta, float)[None, :], rule, value, h, precision)
+    completed = bool(r["completed"][0])
+    info = rank_info(r["ranks_sigma"][0], r["ranks_factor"][0], ctx.n, ctx.b, completed)
+    info["completed"] = completed
+    if not completed:
+        info["failed_at"] = int(r["failed_at"][0])
+        return None, None, info
+    info.update(logdet=float(r["logdet"][0]), quad=float(r["quad"][0]))
+    return float(r["loglik"][0]), r["grad"][0].copy(), info
 
 
 def neg_loglik_tlr(logth, S, y, bounds, rule, value, trace):

@@ -5,8 +5,10 @@ Workload (the BASELINE configuration of the plain TLR Cholesky with recompressio
 i.i.d. uniform in the unit square (default_rng(0)), Morton-ordered, Matern nu=3/2, tiles b = 500
 (80 x 80 tile grid), adaptive rank rule tau = 0.9, ranks re-chosen at each of 20 theta on a Latin
 hypercube (scipy.stats.qmc.LatinHypercube(d=3, seed=0)) over sigma2 in [0.5, 2], range in [0.1, 0.6],
-eta in [0.01, 0.1].  One step = one batched evaluation of those 20 theta (log L, log-det, quadratic
-form, per-tile ranks); value = evaluations per second.
+eta in [0.01, 0.1].  One step = one batched evaluation of log L and its gradient in log theta at those 20
+theta (log L, log-det, quadratic form, per-tile ranks; gradient at frozen ranks: the sigma2 component in
+closed form, range and eta by central differences that replay the recorded ranks, i.e. four further
+evaluations per theta); value = log L + gradient evaluations per second.
 
 Also reported: the roofline of the dominant kernel (the Matern products of the compression, timed with
 HIP events on the library's stream inside the timed region) and a CPU baseline (the NumPy/SciPy oracle
@@ -44,7 +46,7 @@ def load_field(n):
     return a[:, :2].copy(), a[:, 2].copy()
 
 
-def cpu_baseline(S, y, b, theta, rule, value, n_sample):
+def cpu_baseline(S, y, b, theta, rule, value, n_sample, evals_per_point):
     """Time the CPU oracle (NumPy/SciPy restatement of the reference TLR Cholesky) on the first n_sample
     sites of the same field (Morton order keeps them a compact sub-region), same b and rule; scaled to
     the full problem by the ratio of algorithmic work (tile count for the compression and C(nB, 3) for the
@@ -76,16 +78,18 @@ def cpu_baseline(S, y, b, theta, rule, value, n_sample):
         tc = time.perf_counter() - t0
     tf = max(t - tc, 0.0)
     t_full = tc * tiles_f / tiles_s + tf * rec_f / max(rec_s, 1.0)
-    return dict(value=1.0 / t_full, unit="evals/s", cores=cores, kind="port",
-                sample=(f"oracle tlr_loglik at N={n_sample}, b={b}, {rule} {value}, theta={list(theta)}: "
-                        f"{t:.1f} s ({tc:.1f} s compression); scaled to N={len(y)} by tile count "
-                        f"(compression) and C(nB,3) (recompressions): {t_full:.0f} s per evaluation"))
+    return dict(value=1.0 / (evals_per_point * t_full), unit="evals/s", cores=cores, kind="port",
+                sample=(f"oracle tlr_loglik at N={n_sample}, b={b}, {rule} {value}, "
+                        f"theta={[float(x) for x in theta]}: {t:.1f} s ({tc:.1f} s compression); scaled to "
+                        f"N={len(y)} by tile count (compression) and C(nB,3) (recompressions): {t_full:.0f} s "
+                        f"per log L, x{evals_per_point} log L evaluations per log L + gradient (same "
+                        f"frozen-rank difference scheme)"))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=40000)
     ap.add_argument("--b", type=int, default=500)
@@ -93,6 +97,7 @@ def main():
     ap.add_argument("--rule", default="adaptive")
     ap.add_argument("--value", type=float, default=0.9)
     ap.add_argument("--cpu-sample", type=int, default=10000, help="sites of the CPU-baseline sample (0: skip)")
+    ap.add_argument("--no-grad", action="store_true", help="time log L alone")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build (see the sharding notes "
@@ -103,13 +108,20 @@ def main():
     thetas = lhs_thetas(a.batch)
     print(f"# device: {tlrgp.device_info()}", file=sys.stderr, flush=True)
     ctx = tlrgp.TLRContext(S, y, a.b, max_batch=a.batch)
+    grad = not a.no_grad
+
+    def step():
+        if grad:
+            return ctx.loglik_grad_batch(thetas, a.rule, a.value)
+        return ctx.loglik_batch(thetas, a.rule, a.value)
+
     for w in range(a.warmup):
-        r = ctx.loglik_batch(thetas, a.rule, a.value)
+        r = step()
         print(f"# warmup {w}: completed {int(r['completed'].sum())}/{a.batch}", file=sys.stderr, flush=True)
-    ctx.set_timing(1)   # Matern products of the compression: the dominant kernel
+    ctx.set_timing(1)   # Matern products of the compression: the dominant compression kernel
     t0 = time.perf_counter()
     for s in range(a.steps):
-        r = ctx.loglik_batch(thetas, a.rule, a.value)
+        r = step()
         print(f"# step {s}: {time.perf_counter() - t0:.3f} s", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     kms, launches, flops = ctx.get_timing()
@@ -124,16 +136,19 @@ def main():
         "value": evals, "unit": "evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seed-0 uniform sites, Morton order, dense FP64 draw of y)",
-        "config": {"workload": f"TLR log-likelihood, N={a.n}, b={a.b}, {a.rule} tau={a.value}, "
-                               f"{a.batch} Latin-hypercube theta per step, plain FP64",
-                   "n": a.n, "b": a.b, "theta_per_step": a.batch, "precision": "fp64", "gradient": False,
+        "config": {"workload": f"TLR log-likelihood{' + gradient' if grad else ''}, N={a.n}, b={a.b}, "
+                               f"{a.rule} tau={a.value}, {a.batch} Latin-hypercube theta per step, plain FP64",
+                   "n": a.n, "b": a.b, "theta_per_step": a.batch, "precision": "fp64", "gradient": grad,
+                   "gradient_method": "frozen ranks: sigma2 closed form, range/eta central differences h=1e-5"
+                   if grad else None, "mixed_precision": None,
                    "completed": int(r["completed"].sum()), "mean_rank_factor": mean_rank},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": "k_matern_product", "launches": launches, "kernel_ms": kms},
     }
     if a.cpu_sample:
-        out["cpu_baseline"] = cpu_baseline(S, y, a.b, tuple(thetas[0]), a.rule, a.value, a.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(S, y, a.b, tuple(thetas[0]), a.rule, a.value, a.cpu_sample,
+                                           5 if grad else 1)
     print(json.dumps(out), flush=True)
 
 

@@ -72,6 +72,19 @@ int tlr_loglik(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, 
 int tlr_loglik_batch(tlr_ctx* ctx, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
                      double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma, int32_t* ranks_factor);
 
+/* Log-likelihood and its gradient in (log sigma2, log range, log eta) at frozen ranks: the ranks of every
+ * tile after the initial compression and after every recompression are those chosen at theta.  The
+ * sigma2 component is exact (0.5 * quad - N / 2: tiles and budgets scale together with sigma2); the range
+ * and eta components are central differences in log theta with step h (1e-5 recommended), each
+ * evaluation replaying the recorded ranks.  grad is m x 3 (NaN when a perturbed evaluation is not PD).
+ * No counterpart in the reference, which has no gradient (its standard errors use finite differences of
+ * log L with re-chosen ranks, codes/15_pilot_standard_errors.py:21-31). */
+int tlr_loglik_grad_batch(tlr_ctx* ctx, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
+                          double h, double* out, double* grad, int32_t* status, int32_t* failed_at,
+                          int32_t* ranks_sigma, int32_t* ranks_factor);
+int tlr_loglik_grad(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, int32_t precision, double h,
+                    double out3[3], double grad3[3], int32_t* ranks_sigma, int32_t* ranks_factor, int32_t* failed_at);
+
 /* Timing of one kernel class with HIP events on the context's stream, for roofline reports (no
  * counterpart in the reference, which times whole evaluations with time.time(), 17:359-362).
  * kernel_class: 0 off, 1 Matern products, 2 residual, 3 panel QR, 4 rank decision.  Setting it resets the
