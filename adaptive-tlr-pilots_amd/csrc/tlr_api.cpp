@@ -78,6 +78,9 @@ struct tlr_ctx {
   int64_t* d_rlog_off = nullptr;       // nB offsets into a rank log
   int* rep_rank0 = nullptr;            // max_batch records of initial ranks (ntile each)
   int* rep_rlog = nullptr;             // max_batch records of recompression ranks (nlog each)
+  // prediction workspace, grown on demand: N x m cross covariances, alpha, prediction sites, reductions
+  double* pred_buf = nullptr;
+  int64_t pred_cap = 0;
   // timing mode: launches of kernel class timing_kernel are bracketed by events
   int timing_kernel = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -86,6 +89,7 @@ struct tlr_ctx {
   double timed_ms = 0.0, timed_flops = 0.0;
   int64_t timed_launches = 0;
   ~tlr_ctx() {
+    if (pred_buf) hipFree(pred_buf);
     if (h_thr) hipHostFree(h_thr);
     if (h_replay) hipHostFree(h_replay);
     for (hipEvent_t e : ev_pool) hipEventDestroy(e);
@@ -530,6 +534,60 @@ int tlr_loglik_grad(tlr_ctx* ctx, const double theta[3], int32_t rule, double va
                                        ranks_factor);
   if (failed_at) *failed_at = fa;
   return rc;
+}
+
+int tlr_predict(tlr_ctx* c, const double theta[3], int32_t rule, double value, int32_t precision, const double* Sp,
+                int64_t n_pred, double* mu, double* sd, double out3[3]) {
+  if (!c || !theta || !Sp || n_pred < 1 || n_pred > 65535 || !mu || !sd) return set_err(TLR_ERR_ARG, "invalid argument");
+  double o3[3];
+  int32_t st = 0, fa = -1;
+  int rc = run_batch(c, 1, theta, rule, value, precision, nullptr, o3, &st, &fa, nullptr, nullptr);
+  if (rc < 0) return rc;
+  if (out3) memcpy(out3, o3, sizeof(o3));
+  if (st != TLR_OK) return TLR_NOT_PD;
+  const int64_t n = c->n;
+  const int m = (int)n_pred;
+  // workspace: Xc (n x m) | alpha (n) | px, py (m) | mu, ss (m)
+  const int64_t need = n * m + n + 4 * (int64_t)m;
+  if (need > c->pred_cap) {
+    if (c->pred_buf) HIPCHK(hipFree(c->pred_buf));
+    c->pred_buf = nullptr;
+    HIPCHK(hipMalloc(&c->pred_buf, need * sizeof(double)));
+    c->pred_cap = need;
+  }
+  double* Xc = c->pred_buf;
+  double* alpha = Xc + n * m;
+  double* px = alpha + n;
+  double* py = px + m;
+  double* dmu = py + m;
+  double* dss = dmu + m;
+  std::vector<double> hx(m), hy(m);
+  for (int j = 0; j < m; ++j) { hx[j] = Sp[2 * j]; hy[j] = Sp[2 * j + 1]; }
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemcpyAsync(px, hx.data(), m * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(py, hy.data(), m * sizeof(double), hipMemcpyHostToDevice, s));
+  SlotArgs sa = c->args();
+  // alpha = L^{-T} z, z = L^{-1} y left in the slot by the factorization
+  HIPCHK(hipMemcpyAsync(alpha, c->slots[0].p.rhs, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIPCHK(launch_solve(sa, 0, alpha, n, 1, 1, s));
+  MaternParams mp;
+  mp.sigma2 = theta[0];
+  mp.inv_range_sqrt3 = sqrt(3.0) / theta[1];
+  mp.nugget = 0.0;
+  HIPCHK(launch_gen_cross(c->sx, c->sy, n, px, py, m, mp, Xc, n, s));
+  HIPCHK(launch_col_dot(Xc, n, n, m, alpha, dmu, s));          // mu = Kp alpha
+  HIPCHK(launch_solve(sa, 0, Xc, n, m, 0, s));                   // W = L^{-1} Kp^T
+  HIPCHK(launch_col_dot(Xc, n, n, m, nullptr, dss, s));          // column sums of W^2
+  HIPCHK(hipStreamSynchronize(s));
+  std::vector<double> ss(m);
+  HIPCHK(hipMemcpy(mu, dmu, m * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ss.data(), dss, m * sizeof(double), hipMemcpyDeviceToHost));
+  const double v0 = theta[0] * (1.0 + theta[2]);
+  for (int j = 0; j < m; ++j) {
+    const double v = v0 - ss[j];
+    sd[j] = sqrt(v > 1e-12 ? v : 1e-12);
+  }
+  return TLR_OK;
 }
 
 int tlr_loglik(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, int32_t precision, double out3[3],

@@ -77,4 +77,11 @@ hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t 
 hipError_t launch_recompress(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_final(const SlotArgs& sa, int nslot, hipStream_t s);
 
+// solve.hip
+hipError_t launch_solve(const SlotArgs& sa, int slot, double* X, int64_t ldx, int m, int trans, hipStream_t s);
+hipError_t launch_gen_cross(const double* sx, const double* sy, int64_t n, const double* px, const double* py, int m,
+                            MaternParams mp, double* Xc, int64_t ldx, hipStream_t s);
+hipError_t launch_col_dot(const double* X, int64_t ldx, int64_t n, int m, const double* w, double* out,
+                          hipStream_t s);
+
 }  // namespace tlr

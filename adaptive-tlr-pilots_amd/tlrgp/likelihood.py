@@ -58,12 +58,14 @@ def _tile_size_from_bounds(n, bounds):
 
 
 class TLRFactor:
-    """Summary of a completed device-resident TLR factor: per-tile ranks, log-determinant, quadratic form."""
+    """Summary of a completed device-resident TLR factor: per-tile ranks, log-determinant, quadratic form,
+    and the (context, theta, rule) that reproduce it for prediction."""
 
-    def __init__(self, ranks, logdet, quad):
+    def __init__(self, ranks, logdet, quad, ctx=None, theta=None, rule=None, value=None, precision="fp64"):
         self.ranks = ranks
         self.logdet = logdet
         self.quad = quad
+        self.ctx, self.theta, self.rule, self.value, self.precision = ctx, theta, rule, value, precision
 
     def __repr__(self):
         nb = self.ranks.shape[0]
@@ -180,8 +182,23 @@ class TLRContext:
             info["failed_at"] = int(r["failed_at"][0])
             return None, None, info
         info.update(logdet=float(r["logdet"][0]), quad=float(r["quad"][0]))
-        fac = TLRFactor(r["ranks_factor"][0], info["logdet"], info["quad"])
+        fac = TLRFactor(r["ranks_factor"][0], info["logdet"], info["quad"], self, np.asarray(theta, float).copy(),
+                        rule, value, precision)
         return float(r["loglik"][0]), fac, info
+
+    def predict(self, theta, rule, value, Sp, precision="fp64"):
+        """Kriging mean and standard deviation at Sp (n_pred, 2) through the TLR factor at theta
+        (tlr_predict, 17:269-276); None when the factorization fails."""
+        Sp = f64(Sp)
+        if Sp.ndim != 2 or Sp.shape[1] != 2:
+            raise ValueError("Sp must be (n_pred, 2)")
+        mu = np.empty(Sp.shape[0])
+        sd = np.empty(Sp.shape[0])
+        rc = check(lib.tlr_predict(self._ptr, dptr(f64(theta, (3,))), RULES[rule], float(value), PRECISIONS[precision],
+                                   dptr(Sp), Sp.shape[0], dptr(mu), dptr(sd), None))
+        if rc == _lib.TLR_NOT_PD:
+            return None
+        return mu, sd
 
 
 _CTX_CACHE = {}
@@ -225,6 +242,32 @@ def tlr_loglik_grad(S, y, theta, bounds, rule, value, h=1e-5, precision="fp64"):
         return None, None, info
     info.update(logdet=float(r["logdet"][0]), quad=float(r["quad"][0]))
     return float(r["loglik"][0]), r["grad"][0].copy(), info
+
+
+def tlr_predict(S, y, Sp, theta, bounds, fac):
+    """Kriging mean and sd at Sp with the TLR factor fac returned by tlr_loglik (17:269-276)."""
+    if fac is None or fac.ctx is None:
+        raise ValueError("fac must be the factor returned by tlr_loglik")
+    out = fac.ctx.predict(theta, fac.rule, fac.value, Sp, fac.precision)
+    if out is None:
+        raise RuntimeError("the factorization failed at theta")
+    return out
+
+
+def crps_gauss(y, mu, sig):
+    """Mean CRPS of Gaussian predictive distributions (closed form; 11:142-146)."""
+    from scipy.special import erf
+    z = (y - mu) / sig
+    pdf = np.exp(-0.5 * z ** 2) / np.sqrt(2 * np.pi)
+    cdf = 0.5 * (1 + erf(z / np.sqrt(2)))
+    return (sig * (z * (2 * cdf - 1) + 2 * pdf - 1 / np.sqrt(np.pi))).mean()
+
+
+def skill(yp, mu, sig):
+    """RMSE, CRPS and 90% interval coverage of held-out predictions (11:162-165)."""
+    z = np.abs(yp - mu) / sig
+    return dict(rmse=float(np.sqrt(((yp - mu) ** 2).mean())), crps=float(crps_gauss(yp, mu, sig)),
+                cover90=float((z <= 1.6448536).mean()))
 
 
 def neg_loglik_tlr(logth, S, y, bounds, rule, value, trace):

@@ -85,6 +85,14 @@ int tlr_loglik_grad_batch(tlr_ctx* ctx, int32_t m, const double* thetas, int32_t
 int tlr_loglik_grad(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, int32_t precision, double h,
                     double out3[3], double grad3[3], int32_t* ranks_sigma, int32_t* ranks_factor, int32_t* failed_at);
 
+/* Kriging prediction through the TLR factor at theta: replaces tlr_predict (17:269-276) with its solves
+ * tlr_forward / tlr_backward (17:230-256).  The factorization is evaluated at theta first (out3 receives its
+ * loglik, logdet, quad; may be NULL).  Sp is n_pred x 2 row-major (host); mu and sd (n_pred, host) receive
+ * Kp L^{-T} L^{-1} y and sqrt(max(sigma2 (1 + eta) - colsum((L^{-1} Kp^T)^2), 1e-12)).  Returns TLR_NOT_PD
+ * when the factorization fails. */
+int tlr_predict(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, int32_t precision, const double* Sp,
+                int64_t n_pred, double* mu, double* sd, double out3[3]);
+
 /* Timing of one kernel class with HIP events on the context's stream, for roofline reports (no
  * counterpart in the reference, which times whole evaluations with time.time(), 17:359-362).
  * kernel_class: 0 off, 1 Matern products, 2 residual, 3 panel QR, 4 rank decision.  Setting it resets the

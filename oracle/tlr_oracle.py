@@ -384,6 +384,22 @@ def tlr_predict(S, y, Sp, theta, bounds, fac):
     return mu, np.sqrt(np.maximum(var, 1e-12))
 
 
+def crps_gauss(y, mu, sig):
+    """Mean CRPS of Gaussian predictive distributions.  Follows 11:142-146."""
+    from scipy.special import erf
+    z = (y - mu) / sig
+    pdf = np.exp(-0.5 * z ** 2) / np.sqrt(2 * np.pi)
+    cdf = 0.5 * (1 + erf(z / np.sqrt(2)))
+    return (sig * (z * (2 * cdf - 1) + 2 * pdf - 1 / np.sqrt(np.pi))).mean()
+
+
+def skill(yp, mu, sig):
+    """RMSE, CRPS, 90% coverage.  Follows 11:162-165."""
+    z = np.abs(yp - mu) / sig
+    return dict(rmse=float(np.sqrt(((yp - mu) ** 2).mean())), crps=float(crps_gauss(yp, mu, sig)),
+                cover90=float((z <= 1.6448536).mean()))
+
+
 def flops_factorization(ranks0, ranksF, b, nB):
     """Algorithmic flop count of one TLR factorization + forward solve from realised ranks.
 

@@ -4,10 +4,20 @@ Reads (text only, no code is run) the reference's ``results/`` folder:
 
 * ``tlr_cholesky/sim_n{N}_obs.csv`` (N = 10k, 20k, 40k) and ``smoke/sim_n2000_obs.csv``: Morton-ordered
   sites and observations written with ``%.18e`` (an exact float64 round trip), stored here as
-  ``sim_n{N}_obs.npy`` (float64, columns x, y, z);
+  ``sim_n{N}_obs.npy`` (float64, columns x, y, z); the 300 held-out points of the N = 10k and 40k runs
+  as ``sim_n{N}_holdout.npy``;
 * the paired result JSONs: dense and TLR log-likelihoods, stored values, mean/max factor ranks,
   completion flags and failing tiles per rule, the TLR log-determinant of ``kl_tlr.json``, the dense
-  quadratic form of ``bound_terms_n{N}.json``.  Written to ``goldens.json``.
+  quadratic form of ``bound_terms_n{N}.json``, the Nelder-Mead MLE results (theta_hat, objective,
+  evaluation and failure counts) of the N = 10k and 20k runs.  Written to ``goldens.json``;
+* ``mcmc_tlrchol/<rule>/c<k>/``: the first 1200 draws of the reference's random-walk chains at N = 5000
+  (adaptive 0.9 chains 1-4, fixed 1e-4 chains 1 and 4) with their settings, as ``mcmc_n5000_heads.npz``
+  and the ``mcmc_n5000`` entry of ``goldens.json``.
+
+The N = 5000 field of those chains is not stored in the reference; it is regenerated with the oracle's
+restatement of the reference's generator (seed 20261020, 300 held-out points) as ``sim_n5000_obs.npy`` /
+``sim_n5000_holdout.npy``.  The chains themselves pin it: a different field would change their accept
+decisions.
 
 Usage:  python3 tests/golden/make_fixtures.py [/root/reference]
 """
@@ -18,6 +28,7 @@ import sys
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+HEAD = 1200   # draws kept per reference chain: covers the Robbins-Monro phase and the re-shaping at 1000
 
 
 def _load_json(ref, rel):
@@ -32,16 +43,21 @@ def main(ref="/root/reference"):
         assert a.shape == (n, 3), a.shape
         np.save(os.path.join(HERE, f"sim_n{n}_obs.npy"), a)
 
+    for n in (10000, 40000):   # held-out points of the same runs (300 each)
+        a = np.loadtxt(os.path.join(ref, "results", f"tlr_cholesky/sim_n{n}_holdout.csv"), delimiter=",", skiprows=1)
+        np.save(os.path.join(HERE, f"sim_n{n}_holdout.npy"), a)
+
     g = dict(source="reference results/ (seed 20261020, theta=(1, 0.3, 0.05), Matern nu=3/2, b=250)", tlr_cholesky={})
     keep = ("rule", "value", "completed", "stored_sigma", "stored_factor", "mean_rank_factor", "max_rank_factor",
-            "loglik", "loglik_err", "failed_at_tile")
+            "loglik", "loglik_err", "failed_at_tile", "skill_true_theta")
     for n in (10000, 20000, 40000):
         d = _load_json(ref, f"tlr_cholesky/tlr_cholesky_n{n}.json")
         rules = [{k: r[k] for k in keep if k in r} for r in d["rules"]]
         if n == 40000:   # the eps rerun adds fixed 1e-5 / 1e-6 at N=40k
             e = _load_json(ref, "tlr_cholesky_eps/tlr_cholesky_n40000.json")
             rules += [{k: r[k] for k in keep if k in r} for r in e["rules"] if r["rule"] == "fixed"]
-        g["tlr_cholesky"][str(n)] = dict(b=d["settings"]["b"], dense_loglik=d["dense_loglik_true"], rules=rules)
+        g["tlr_cholesky"][str(n)] = dict(b=d["settings"]["b"], dense_loglik=d["dense_loglik_true"], rules=rules,
+                                         dense_skill=d.get("dense_skill_true_theta"))
     kl = _load_json(ref, "screening_stress/nu15_r03_e005/kl_tlr.json")
     g["kl_tlr_n10000"] = dict(logdet_exact=kl["logdet_exact"], logdet_approx=kl["logdet_approx"],
                               tau=kl["tau"], note="screening field nu15_r03_e005 = the same N=10k field")
@@ -49,6 +65,30 @@ def main(ref="/root/reference"):
     for n in (10000, 40000):
         bt = _load_json(ref, f"bound_terms/bound_terms_n{n}.json")
         g["bound_terms"][str(n)] = dict(dense_loglik=bt["dense_loglik"], z2=bt["z2"])
+    g["mle"] = {}
+    for n in (10000, 20000):
+        d = _load_json(ref, f"tlr_cholesky/tlr_cholesky_n{n}.json")
+        g["mle"][str(n)] = dict(x0_mult=[1.5, 0.6, 2.0], theta_true=d["theta_true"],
+                                fits=[{k: f[k] for k in ("rule", "value", "theta_hat", "neg_loglik", "evals",
+                                                         "failures", "skill") if k in f} for f in d["mle"]])
+    chains, meta = {}, []
+    for rd, cs in (("adaptive_0.9", (1, 2, 3, 4)), ("fixed_1e-4", (1, 4))):
+        for c in cs:
+            m = _load_json(ref, f"mcmc_tlrchol/{rd}/c{c}/tlr_mcmc_n5000.json")
+            key = f"{rd}_c{c}"
+            chains[key] = np.load(os.path.join(ref, "results", f"mcmc_tlrchol/{rd}/c{c}/chain_n5000.npy"))[:HEAD]
+            st = m["settings"]
+            meta.append(dict(key=key, rule=m["rule"], value=m["value"], chain_seed=st["chain_seed"],
+                             start_theta=m["start_theta"], start_mult=[float(v) for v in st["start_mult"].split(",")],
+                             iters=st["iters"], burn=st["burn"], b=st["b"], acceptance=m["acceptance"],
+                             factorization_failures=m["factorization_failures"]))
+    np.savez(os.path.join(HERE, "mcmc_n5000_heads.npz"), **chains)
+    g["mcmc_n5000"] = dict(head=HEAD, theta_true=[1.0, 0.3, 0.05], seed=20261020, n_pred=300, chains=meta)
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(HERE)), "oracle"))
+    import tlr_oracle as O
+    S, y, Sp, yp = O.simulate_reference(5000)
+    np.save(os.path.join(HERE, "sim_n5000_obs.npy"), np.column_stack([S, y]))
+    np.save(os.path.join(HERE, "sim_n5000_holdout.npy"), np.column_stack([Sp, yp]))
     sm = _load_json(ref, "smoke/tlr_rank_rules_n2000.json")
     g["smoke_rank_rules_n2000"] = [dict(rule=r["rule"], value=r["value"], mean_rank=r["mean_rank"],
                                         max_rank=r["max_rank"], min_rank=r["min_rank"], pd=r["pd"],

@@ -1,0 +1,62 @@
+"""GPU kriging through the TLR factor (tlr_predict, tlr_forward/tlr_backward of codes/17) against the
+oracle elementwise and against the reference's committed skill_true_theta (RMSE, CRPS, 90% coverage).
+Tolerances: mean and sd 1e-9 relative to the oracle; skill scores 1e-8 absolute (the reference's
+checker uses 1e-6 relative for fixed-theta quantities)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _field(n):
+    a = np.load(os.path.join(GOLDEN, f"sim_n{n}_obs.npy"))
+    return a[:, :2].copy(), a[:, 2].copy()
+
+
+@pytest.fixture(scope="module")
+def tlrgp():
+    import tlrgp as m
+    return m
+
+
+@pytest.fixture(scope="module")
+def goldens():
+    with open(os.path.join(GOLDEN, "goldens.json")) as f:
+        return json.load(f)
+
+
+def test_predict_vs_oracle_small(tlrgp):
+    import tlr_oracle as O
+    S, y = _field(2000)
+    Sp = np.random.default_rng(5).random((77, 2))
+    th = np.array([1.3, 0.2, 0.03])
+    bnd = tlrgp.tile_bounds(2000, 250)
+    print("case predict N=2000 b=250 adaptive:0.9", flush=True)
+    ll, fac, info = tlrgp.tlr_loglik(S, y, th, bnd, "adaptive", 0.9)
+    mu, sd = tlrgp.tlr_predict(S, y, Sp, th, bnd, fac)
+    ll_o, fac_o, _ = O.tlr_loglik(S, y, th, bnd, "adaptive", 0.9)
+    mu_o, sd_o = O.tlr_predict(S, y, Sp, th, bnd, fac_o)
+    np.testing.assert_allclose(mu, mu_o, rtol=1e-9, atol=1e-10)
+    np.testing.assert_allclose(sd, sd_o, rtol=1e-9, atol=1e-10)
+
+
+@pytest.mark.parametrize("n", [10000, 40000])
+def test_predict_skill_goldens(tlrgp, goldens, n):
+    S, y = _field(n)
+    hold = np.load(os.path.join(GOLDEN, f"sim_n{n}_holdout.npy"))
+    th = np.array([1.0, 0.3, 0.05])
+    bnd = tlrgp.tile_bounds(n, 250)
+    for r in goldens["tlr_cholesky"][str(n)]["rules"]:
+        if not r.get("completed") or "skill_true_theta" not in r:
+            continue
+        print(f"case predict N={n} b=250 {r['rule']}:{r['value']}", flush=True)
+        ll, fac, info = tlrgp.tlr_loglik(S, y, th, bnd, r["rule"], r["value"])
+        mu, sd = tlrgp.tlr_predict(S, y, hold[:, :2], th, bnd, fac)
+        sk = tlrgp.skill(hold[:, 2], mu, sd)
+        for key in ("rmse", "crps", "cover90"):
+            assert abs(sk[key] - r["skill_true_theta"][key]) < 1e-8, (r["rule"], r["value"], key, sk[key])

@@ -156,3 +156,18 @@ def test_n10000_rules_match_reference(goldens):
                 assert abs(info["logdet"] - goldens["kl_tlr_n10000"]["logdet_approx"]) < 1e-6 * 3e4
         else:
             assert info["failed_at"] == r["failed_at_tile"]
+
+
+@pytest.mark.slow
+def test_n10000_predict_skill_matches_reference(goldens):
+    """tlr_predict + skill at the true theta against skill_true_theta of the reference run (adaptive 0.9)."""
+    S, y = _golden_field(10000)
+    hold = np.load(os.path.join(GOLDEN, "sim_n10000_holdout.npy"))
+    th = np.array([1.0, 0.3, 0.05])
+    bnd = O.tile_bounds(10000, 250)
+    ll, fac, info = O.tlr_loglik(S, y, th, bnd, "adaptive", 0.9)
+    mu, sd = O.tlr_predict(S, y, hold[:, :2], th, bnd, fac)
+    sk = O.skill(hold[:, 2], mu, sd)
+    ref = [r for r in goldens["tlr_cholesky"]["10000"]["rules"] if r["rule"] == "adaptive"][0]["skill_true_theta"]
+    for key in ("rmse", "crps", "cover90"):
+        assert abs(sk[key] - ref[key]) < 1e-9, (key, sk[key], ref[key])
