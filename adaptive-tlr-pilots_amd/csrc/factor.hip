@@ -382,45 +382,67 @@ __global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
 //   Unew = [Q1,   Q2u] [[D, Cu], [0, R2u]],  Cu = Q1^T X,           X - Q1 Cu      = Q2u R2u
 // (block Gram-Schmidt applied twice, Householder QR of the r_jk new columns only).  The core
 // R_u R_v^T (w x w, w = r_ij + r_jk) goes to the one-sided Jacobi SVD; the outputs are formed row block
-// by row block through LDS, in place.  The panel holds the r_jk new columns: variant NW=4 for
-// r_jk <= 32, NW=8 for r_jk <= 64; both variants see every pair and keep the ones of their class.
-template <int NW>
+// by row block through LDS, in place.  The panel holds the r_jk new columns in NW waves x CPW column
+// slots: r_jk <= 8 (nine pairs in ten) in 4 waves x 2 slots with a 43 KB LDS footprint, so that three
+// work groups share a CU; r_jk <= 16 in 4 x 4 (two per CU); r_jk <= 32 in 4 x 8; r_jk <= 64 in 8 x 8
+// (their cores up to 64 x 64 in LDS).  The pairs of each class come from a work list.
+template <int NW, int CPW>
 struct RecompLayout {
-  static constexpr int R2MAX = 8 * NW;
+  static constexpr int R2MAX = CPW * NW;
   static constexpr int WMAX = 64 + R2MAX;
-  static constexpr int WJ = 48;                                   // cores up to 48 x 48 run in LDS
+  // cores up to WJ x WJ run in LDS (larger ones in the work group's scratch)
+  static constexpr int WJ = R2MAX <= 16 ? 32 : 64;
   static constexpr bool RR_LDS = WMAX * R2MAX <= 3072;
-  static constexpr int FRONT = (2 * 64 * R2MAX > 64 * WMAX) ? 2 * 64 * R2MAX : 64 * WMAX;
+  // front: W and the CGS coefficients (2 x 64 x R2MAX), later the output staging (RB rows x w)
+  static constexpr int FRONT = 2 * 64 * R2MAX > 2048 ? 2 * 64 * R2MAX : 2048;
   static constexpr int RR = RR_LDS ? WMAX * R2MAX : 0;
   static constexpr int OFF_RV = FRONT, OFF_RU = OFF_RV + RR, OFF_SC = OFF_RU + RR, OFF_SG = OFF_SC + 64,
                        OFF_G = OFF_SG + 64, OFF_J = OFF_G + WJ * WJ, TOTAL = OFF_J + WJ * WJ;
+  static constexpr int JTP = NW >= 8 ? 8 : 4;   // threads per Jacobi column pair (64 pairs: w <= 128)
 };
 
-template <int RPL, int NW>
-__global__ __launch_bounds__(64 * NW) void k_recompress(SlotArgs sa, int k, int r2lo, int r2hi) {
-  using L = RecompLayout<NW>;
-  const int slot = blockIdx.z;
+__device__ __forceinline__ int recomp_class(int r2) { return r2 <= 8 ? 0 : (r2 <= 16 ? 1 : (r2 <= 32 ? 2 : 3)); }
+
+// Work lists of step k: one thread per (pair, slot); pairs with r_ik, r_jk > 0 go to the list of their
+// class.  Ranks outside [0, rank_cap] trip the guard instead.  The order inside a list depends on the
+// atomics; every pair is computed independently of the others, so results do not.
+__global__ __launch_bounds__(256) void k_recomp_list(SlotArgs sa, int k, int npairs) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  const int slot = blockIdx.y;
+  if (q >= npairs) return;
   const SlotPtrs P = sa.slot(slot);
   if (failed(P.status)) return;
-  const int b = sa.b;
   int a_, b_;
-  tri_decode((int64_t)blockIdx.x, &a_, &b_);   // pairs (a, b'), 0 <= b' < a; index 0 is (1, 0)
+  tri_decode((int64_t)q, &a_, &b_);   // pairs (a, b'), 0 <= b' < a; index 0 is (1, 0)
   const int i = a_ + k + 1, j = b_ + k + 1;
-  if (i >= sa.nB) return;
-  const int64_t tij = tri_index(i, j), tik = tri_index(i, k), tjk = tri_index(j, k);
-  const int r1 = P.rank[tij], rik = P.rank[tik], r2 = P.rank[tjk];
-  if (rik == 0 || r2 == 0 || r2 < r2lo || r2 > r2hi) return;
-  if (r1 < 0 || r1 > sa.rank_cap || rik < 0 || rik > sa.rank_cap) {
-    if (threadIdx.x == 0) atomicMax(&P.status[ST_GUARD], 1);
+  const int r1 = P.rank[tri_index(i, j)], rik = P.rank[tri_index(i, k)], r2 = P.rank[tri_index(j, k)];
+  if (r1 < 0 || r1 > sa.rank_cap || rik < 0 || rik > sa.rank_cap || r2 < 0 || r2 > sa.rank_cap) {
+    atomicMax(&P.status[ST_GUARD], 1);
     return;
   }
+  if (rik == 0 || r2 == 0) return;
+  const int c = recomp_class(r2);
+  const int pos = atomicAdd(&sa.wl_count[RCLASSES * k + c], 1);
+  sa.wl[(int64_t)c * sa.wl_cap + pos] = make_int2(q, slot);
+}
+
+// One pair (i, j) = pair index q of step k, for slot `slot`; scr is the work group's scratch.
+template <int RPL, int NW, int CPW>
+__device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q, int slot, double* scr) {
+  using L = RecompLayout<NW, CPW>;
+  const SlotPtrs P = sa.slot(slot);
+  const int b = sa.b;
+  int a_, b_;
+  tri_decode((int64_t)q, &a_, &b_);
+  const int i = a_ + k + 1, j = b_ + k + 1;
+  const int64_t tij = tri_index(i, j), tik = tri_index(i, k), tjk = tri_index(j, k);
+  const int r1 = P.rank[tij], rik = P.rank[tik], r2 = P.rank[tjk];
   const int w = r1 + r2;
   const double thr_rel = sa.thr[2 * slot], thr_abs = sa.thr[2 * slot + 1];
   extern __shared__ double lds[];
   double* Wm = lds;                           // rik x r2          (phases 1-3)
   double* Ct = lds + 64 * L::R2MAX;           // r1 x r2 temporary (phases 2-3)
-  double* Sb = lds;                           // 64 x w staging     (phase 5, aliases the front)
-  double* scr = P.scratch + (int64_t)blockIdx.x * sa.scr_stride;
+  double* Sb = lds;                           // RB x w staging     (phase 5, aliases the front)
   double* Q2v = scr;                          // b x r2
   double* RvR = L::RR_LDS ? lds + L::OFF_RV : scr + (int64_t)b * 64;            // w x r2: [Cv; R2v]
   double* RuR = L::RR_LDS ? lds + L::OFF_RU : scr + (int64_t)b * 64 + 128 * 64; // w x r2: [Cu; R2u]
@@ -442,7 +464,7 @@ __global__ __launch_bounds__(64 * NW) void k_recompress(SlotArgs sa, int k, int 
   const double* Vik = P.V + tik * fs;
   const double* Ujk = P.U + tjk * fs;
   const double* Vjk = P.V + tjk * fs;
-  RegPanel<RPL, NW> pan;
+  RegPanel<RPL, NW, CPW> pan;
 
   // ---- 1. W = V_ik^T V_jk
   pan.load(Vjk, b, b, r2);
@@ -492,7 +514,7 @@ __global__ __launch_bounds__(64 * NW) void k_recompress(SlotArgs sa, int k, int 
     G[a + c * w] = s;
   }
   __syncthreads();
-  const int sweeps = jacobi_svd<(NW >= 8 ? 8 : 4)>(G, w, J, w, w, sig, perm, &flag, 60);
+  const int sweeps = jacobi_svd<L::JTP>(G, w, J, w, w, sig, perm, &flag, 60);
   if (sweeps == -2) {   // non-finite core: report, keep the tile unchanged
     if (tid == 0) atomicMax(&P.status[ST_GUARD], 2);
     return;
@@ -508,7 +530,7 @@ __global__ __launch_bounds__(64 * NW) void k_recompress(SlotArgs sa, int k, int 
       const double nacc = acc + s * s;
       if (sqrt(nacc) <= thr) { acc = nacc; r = c; } else break;
     }
-    const int64_t lo = sa.rlog_off[k] + blockIdx.x;
+    const int64_t lo = sa.rlog_off[k] + q;
     const int rep = sa.replay[slot];
     if (rep >= 0) r = min(sa.rep_rlog[(int64_t)rep * sa.nlog + lo], w);   // frozen ranks
     if (r > sa.rank_cap) { atomicMax(&P.status[ST_RANKCAP], 1); r = sa.rank_cap; }
@@ -517,47 +539,74 @@ __global__ __launch_bounds__(64 * NW) void k_recompress(SlotArgs sa, int k, int 
   }
   __syncthreads();
   const int rn = s_rank;
-  // ---- 5. outputs, row blocks of 64 staged through LDS (in place)
+  // ---- 5. outputs, row blocks of RB rows staged through LDS (in place); register row block t of the
+  // panel covers rows [64 t, 64 t + 64), so t stays a compile-time index
   pan.org2r(b, r2, vbuf, tau);   // Q2u in registers
+  const int rbs = 64 * w <= L::FRONT ? 6 : (32 * w <= L::FRONT ? 5 : 4);
+  const int RB = 1 << rbs;
 #pragma unroll
   for (int t = 0; t < RPL; ++t) {
-    const int r0 = 64 * t;
-    if (r0 >= b) break;
-    // V_ij <- [V_ij, Q2v] J[:, perm]
-    for (int e = tid; e < 64 * w; e += NT) {
-      const int rr = e % 64, m = e / 64, p = r0 + rr;
-      Sb[m * 64 + rr] = (p < b) ? (m < r1 ? Vij[p + (int64_t)m * b] : Q2v[p + (int64_t)(m - r1) * b]) : 0.0;
-    }
-    __syncthreads();
-    for (int e = tid; e < 64 * rn; e += NT) {
-      const int rr = e % 64, c = e / 64, p = r0 + rr;
-      const double* Jc = J + perm[c] * w;
-      double s = 0.0;
-      for (int m = 0; m < w; ++m) s = fma(Sb[m * 64 + rr], Jc[m], s);
-      if (p < b) Vij[p + (int64_t)c * b] = s;
-    }
-    __syncthreads();
-    // U_ij <- [Q1, Q2u] G[:, perm],  Q1 = U_ij diag(1/sigma)
-    for (int e = tid; e < 64 * r1; e += NT) {
-      const int rr = e % 64, m = e / 64, p = r0 + rr;
-      Sb[m * 64 + rr] = (p < b) ? Uij[p + (int64_t)m * b] * sc[m] : 0.0;
-    }
+    if (64 * t >= b) break;
+    const int rend = min(64 * t + 64, b);
+    for (int r0 = 64 * t; r0 < rend; r0 += RB) {
+      // V_ij <- [V_ij, Q2v] J[:, perm]
+      for (int e = tid; e < RB * w; e += NT) {
+        const int rr = e & (RB - 1), m = e >> rbs, p = r0 + rr;
+        Sb[m * RB + rr] = (p < b) ? (m < r1 ? Vij[p + (int64_t)m * b] : Q2v[p + (int64_t)(m - r1) * b]) : 0.0;
+      }
+      __syncthreads();
+      for (int e = tid; e < RB * rn; e += NT) {
+        const int rr = e & (RB - 1), c = e >> rbs, p = r0 + rr;
+        const double* Jc = J + perm[c] * w;
+        double s = 0.0;
+        for (int m = 0; m < w; ++m) s = fma(Sb[m * RB + rr], Jc[m], s);
+        if (p < b) Vij[p + (int64_t)c * b] = s;
+      }
+      __syncthreads();
+      // U_ij <- [Q1, Q2u] G[:, perm],  Q1 = U_ij diag(1/sigma)
+      for (int e = tid; e < RB * r1; e += NT) {
+        const int rr = e & (RB - 1), m = e >> rbs, p = r0 + rr;
+        Sb[m * RB + rr] = (p < b) ? Uij[p + (int64_t)m * b] * sc[m] : 0.0;
+      }
+      const int rr_own = ln - (r0 - 64 * t);
 #pragma unroll
-    for (int jj = 0; jj < 8; ++jj) {
-      const int c = RegPanel<RPL, NW>::col_of(wv, jj);
-      if (c < r2) Sb[(r1 + c) * 64 + ln] = pan.a[jj][t];
+      for (int jj = 0; jj < CPW; ++jj) {
+        const int c = RegPanel<RPL, NW, CPW>::col_of(wv, jj);
+        if (c < r2 && rr_own >= 0 && rr_own < RB) Sb[(r1 + c) * RB + rr_own] = pan.a[jj][t];
+      }
+      __syncthreads();
+      for (int e = tid; e < RB * rn; e += NT) {
+        const int rr = e & (RB - 1), c = e >> rbs, p = r0 + rr;
+        const double* Gc = G + perm[c] * w;
+        double s = 0.0;
+        for (int m = 0; m < w; ++m) s = fma(Sb[m * RB + rr], Gc[m], s);
+        if (p < b) Uij[p + (int64_t)c * b] = s;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    for (int e = tid; e < 64 * rn; e += NT) {
-      const int rr = e % 64, c = e / 64, p = r0 + rr;
-      const double* Gc = G + perm[c] * w;
-      double s = 0.0;
-      for (int m = 0; m < w; ++m) s = fma(Sb[m * 64 + rr], Gc[m], s);
-      if (p < b) Uij[p + (int64_t)c * b] = s;
-    }
-    __syncthreads();
   }
   if (tid == 0) P.rank[tij] = rn;
+}
+
+// Persistent grid over the class-cls work list of step k (grid = resident work groups of the class).
+// The narrow classes ask for 3 (r_jk <= 8) and 2 (r_jk <= 16) waves per SIMD so that the register
+// allocation leaves room for as many work groups as their LDS footprint allows.
+template <int NW, int CPW>
+struct RecompWaves {
+  static constexpr int value = NW * CPW <= 8 ? 3 : (NW * CPW <= 16 ? 2 : 1);
+};
+
+template <int RPL, int NW, int CPW>
+__global__ __launch_bounds__(64 * NW, (RecompWaves<NW, CPW>::value)) void k_recompress(SlotArgs sa, int k, int cls) {
+  const int cnt = __builtin_amdgcn_readfirstlane(sa.wl_count[RCLASSES * k + cls]);
+  const int2* list = sa.wl + (int64_t)cls * sa.wl_cap;
+  double* scr = sa.scr_pool + (int64_t)blockIdx.x * sa.scr_stride;
+  for (int q = blockIdx.x; q < cnt; q += gridDim.x) {
+    const int2 e = list[q];
+    recompress_pair<RPL, NW, CPW>(sa, k, __builtin_amdgcn_readfirstlane(e.x), __builtin_amdgcn_readfirstlane(e.y),
+                                  scr);
+    __syncthreads();
+  }
 }
 
 // ------------------------------------------------------------------------------------ final
@@ -606,36 +655,77 @@ hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t 
   return hipGetLastError();
 }
 
-template <int RPL, int NW>
-static hipError_t launch_recomp_t(const SlotArgs& sa, int k, int npairs, int nslot, int r2lo, int r2hi, hipStream_t s) {
-  const size_t lds = (size_t)RecompLayout<NW>::TOTAL * sizeof(double);
+template <int RPL, int NW, int CPW>
+static hipError_t recomp_attr() {
   static bool attr_set = false;
+  hipError_t e = hipSuccess;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void*)k_recompress<RPL, NW>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-    attr_set = true;
+    e = hipFuncSetAttribute((const void*)k_recompress<RPL, NW, CPW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(RecompLayout<NW, CPW>::TOTAL * sizeof(double)));
+    attr_set = e == hipSuccess;
   }
-  hipLaunchKernelGGL((k_recompress<RPL, NW>), dim3(npairs, 1, nslot), dim3(64 * NW), lds, s, sa, k, r2lo, r2hi);
+  return e;
+}
+
+template <int RPL, int NW, int CPW>
+static hipError_t launch_recomp_t(const SlotArgs& sa, int k, int cls, int grid, hipStream_t s) {
+  hipError_t e = recomp_attr<RPL, NW, CPW>();
+  if (e != hipSuccess) return e;
+  const size_t lds = (size_t)RecompLayout<NW, CPW>::TOTAL * sizeof(double);
+  hipLaunchKernelGGL((k_recompress<RPL, NW, CPW>), dim3(grid), dim3(64 * NW), lds, s, sa, k, cls);
   return hipGetLastError();
 }
 
-hipError_t launch_recompress(const SlotArgs& sa, int k, int nslot, hipStream_t s) {
+template <int RPL, int NW, int CPW>
+static hipError_t occ_t(int* out) {
+  hipError_t e = recomp_attr<RPL, NW, CPW>();
+  if (e != hipSuccess) return e;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, (const void*)k_recompress<RPL, NW, CPW>, 64 * NW,
+                                                      RecompLayout<NW, CPW>::TOTAL * sizeof(double));
+}
+
+hipError_t recompress_occupancy(int b, int blocks_per_cu[RCLASSES]) {
+  hipError_t e;
+  if (b <= 256) {
+    if ((e = occ_t<4, 4, 2>(blocks_per_cu + 0)) != hipSuccess) return e;
+    if ((e = occ_t<4, 4, 4>(blocks_per_cu + 1)) != hipSuccess) return e;
+    if ((e = occ_t<4, 4, 8>(blocks_per_cu + 2)) != hipSuccess) return e;
+    return occ_t<4, 8, 8>(blocks_per_cu + 3);
+  }
+  if ((e = occ_t<8, 4, 2>(blocks_per_cu + 0)) != hipSuccess) return e;
+  if ((e = occ_t<8, 4, 4>(blocks_per_cu + 1)) != hipSuccess) return e;
+  if ((e = occ_t<8, 4, 8>(blocks_per_cu + 2)) != hipSuccess) return e;
+  return occ_t<8, 8, 8>(blocks_per_cu + 3);
+}
+
+// Work lists of step k (one thread per pair and slot); false when step k has no pairs.
+hipError_t launch_recomp_list(const SlotArgs& sa, int k, int nslot, hipStream_t s) {
   const int m = sa.nB - k - 1;            // tiles below the panel
   const int npairs = m * (m - 1) / 2;     // pairs (i, j), k < j < i
   if (npairs <= 0) return hipSuccess;
-  // r_jk <= 32: 4 waves; up to 64: 8 waves (the class depends only on the column-k rank, which this
-  // step does not change, so each pair is handled by exactly one launch)
-  hipError_t e;
+  hipLaunchKernelGGL(k_recomp_list, dim3((npairs + 255) / 256, nslot), dim3(256), 0, s, sa, k, npairs);
+  return hipGetLastError();
+}
+
+// One persistent launch of class cls for step k (grid work groups, each looping over the class list;
+// scr is the class's scratch pool, grid x scr_stride doubles).
+hipError_t launch_recomp_class(const SlotArgs& sa_in, int k, int cls, int grid, double* scr, hipStream_t s) {
+  SlotArgs sa = sa_in;
+  sa.scr_pool = scr;
   if (sa.b <= 256) {
-    e = launch_recomp_t<4, 4>(sa, k, npairs, nslot, 1, 32, s);
-    if (e != hipSuccess) return e;
-    e = launch_recomp_t<4, 8>(sa, k, npairs, nslot, 33, 64, s);
-  } else {
-    e = launch_recomp_t<8, 4>(sa, k, npairs, nslot, 1, 32, s);
-    if (e != hipSuccess) return e;
-    e = launch_recomp_t<8, 8>(sa, k, npairs, nslot, 33, 64, s);
+    switch (cls) {
+      case 0: return launch_recomp_t<4, 4, 2>(sa, k, 0, grid, s);
+      case 1: return launch_recomp_t<4, 4, 4>(sa, k, 1, grid, s);
+      case 2: return launch_recomp_t<4, 4, 8>(sa, k, 2, grid, s);
+      default: return launch_recomp_t<4, 8, 8>(sa, k, 3, grid, s);
+    }
   }
-  return e;
+  switch (cls) {
+    case 0: return launch_recomp_t<8, 4, 2>(sa, k, 0, grid, s);
+    case 1: return launch_recomp_t<8, 4, 4>(sa, k, 1, grid, s);
+    case 2: return launch_recomp_t<8, 4, 8>(sa, k, 2, grid, s);
+    default: return launch_recomp_t<8, 8, 8>(sa, k, 3, grid, s);
+  }
 }
 
 hipError_t launch_final(const SlotArgs& sa, int nslot, hipStream_t s) {

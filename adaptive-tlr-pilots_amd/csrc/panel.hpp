@@ -1,6 +1,6 @@
 // Work-group-wide small dense linear algebra kept in registers / LDS:
-//  * RegPanel: an m x n panel (m <= 64*RPL, n <= 8*NW) distributed over NW waves; wave w owns the
-//    columns j = w + NW*jj (jj < 8), lane l owns the rows p = l + 64*t (t < RPL).  Householder QR with
+//  * RegPanel: an m x n panel (m <= 64*RPL, n <= CPW*NW) distributed over NW waves; wave w owns the
+//    columns j = w + NW*jj (jj < CPW), lane l owns the rows p = l + 64*t (t < RPL).  Householder QR with
 //    LAPACK dlarfg/dgeqr2 conventions, explicit Q in place (dorg2r), all without spilling the panel.
 //  * panel_coef / panel_axpy: projections of the panel against a global matrix (block Gram-Schmidt).
 //  * jacobi_svd: one-sided (Hestenes) Jacobi on the columns of a small square matrix, round-robin
@@ -10,9 +10,9 @@
 
 namespace tlr {
 
-template <int RPL, int NW>
+template <int RPL, int NW, int CPW_ = 8>
 struct RegPanel {
-  static constexpr int CPW = 8;
+  static constexpr int CPW = CPW_;
   static constexpr int NT = 64 * NW;
   double a[CPW][RPL];
 
@@ -189,53 +189,73 @@ struct RegPanel {
 };
 
 // C[a + j*ldc] = sc[a] * (A[:, a]^T P[:, j]) for a < na, j < n; A global (m x na, column-major, lda),
-// sc optional column scale.  Each wave produces the entries of its own panel columns.
-template <int RPL, int NW>
-__device__ __forceinline__ void panel_coef(const RegPanel<RPL, NW>& P, const double* A, int64_t lda, int m, int na,
-                                           int n, const double* sc, double* C, int ldc) {
+// sc optional column scale.  Each wave produces the entries of its own panel columns.  The columns of A
+// are fetched AU at a time so that several loads are in flight per wave.
+template <int RPL, int NW, int CPW>
+__device__ __forceinline__ void panel_coef(const RegPanel<RPL, NW, CPW>& P, const double* A, int64_t lda, int m,
+                                           int na, int n, const double* sc, double* C, int ldc) {
+  constexpr int AU = CPW <= 4 ? 2 : 1;
   const int w = wave_id(), l = lane_id();
-  for (int a = 0; a < na; ++a) {
-    double av[RPL];
+  for (int a0 = 0; a0 < na; a0 += AU) {
+    double av[AU][RPL];
 #pragma unroll
-    for (int t = 0; t < RPL; ++t) {
-      const int p = l + 64 * t;
-      av[t] = (p < m) ? A[p + (int64_t)a * lda] : 0.0;
-    }
-    const double s = sc ? sc[a] : 1.0;
+    for (int u = 0; u < AU; ++u)
 #pragma unroll
-    for (int jj = 0; jj < 8; ++jj) {
-      const int j = RegPanel<RPL, NW>::col_of(w, jj);
-      if (j < n) {
-        double d = 0.0;
+      for (int t = 0; t < RPL; ++t) {
+        const int p = l + 64 * t;
+        av[u][t] = (a0 + u < na && p < m) ? A[p + (int64_t)(a0 + u) * lda] : 0.0;
+      }
 #pragma unroll
-        for (int t = 0; t < RPL; ++t) d += av[t] * P.a[jj][t];
-        d = wave_sum(d);
-        if (l == 0) C[a + j * ldc] = d * s;
+    for (int u = 0; u < AU; ++u) {
+      const int a = a0 + u;
+      if (a < na) {
+        const double s = sc ? sc[a] : 1.0;
+#pragma unroll
+        for (int jj = 0; jj < CPW; ++jj) {
+          const int j = RegPanel<RPL, NW, CPW>::col_of(w, jj);
+          if (j < n) {
+            double d = 0.0;
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) d += av[u][t] * P.a[jj][t];
+            d = wave_sum(d);
+            if (l == 0) C[a + j * ldc] = d * s;
+          }
+        }
       }
     }
   }
 }
 
 // P[:, j] += alpha * sum_a A[:, a] sc[a] C[a + j*ldc] for j < n (C must be visible to every wave).
-template <int RPL, int NW>
-__device__ __forceinline__ void panel_axpy(RegPanel<RPL, NW>& P, const double* A, int64_t lda, int m, int na, int n,
-                                           const double* sc, const double* C, int ldc, double alpha) {
+// Columns of A fetched AU at a time; the accumulation order over a is sequential.
+template <int RPL, int NW, int CPW>
+__device__ __forceinline__ void panel_axpy(RegPanel<RPL, NW, CPW>& P, const double* A, int64_t lda, int m, int na,
+                                           int n, const double* sc, const double* C, int ldc, double alpha) {
+  constexpr int AU = CPW <= 4 ? 2 : 1;
   const int w = wave_id(), l = lane_id();
-  for (int a = 0; a < na; ++a) {
-    double av[RPL];
+  for (int a0 = 0; a0 < na; a0 += AU) {
+    double av[AU][RPL];
 #pragma unroll
-    for (int t = 0; t < RPL; ++t) {
-      const int p = l + 64 * t;
-      av[t] = (p < m) ? A[p + (int64_t)a * lda] : 0.0;
-    }
-    const double s = alpha * (sc ? sc[a] : 1.0);
+    for (int u = 0; u < AU; ++u)
 #pragma unroll
-    for (int jj = 0; jj < 8; ++jj) {
-      const int j = RegPanel<RPL, NW>::col_of(w, jj);
-      if (j < n) {
-        const double f = s * C[a + j * ldc];
+      for (int t = 0; t < RPL; ++t) {
+        const int p = l + 64 * t;
+        av[u][t] = (a0 + u < na && p < m) ? A[p + (int64_t)(a0 + u) * lda] : 0.0;
+      }
 #pragma unroll
-        for (int t = 0; t < RPL; ++t) P.a[jj][t] = fma(f, av[t], P.a[jj][t]);
+    for (int u = 0; u < AU; ++u) {
+      const int a = a0 + u;
+      if (a < na) {
+        const double s = alpha * (sc ? sc[a] : 1.0);
+#pragma unroll
+        for (int jj = 0; jj < CPW; ++jj) {
+          const int j = RegPanel<RPL, NW, CPW>::col_of(w, jj);
+          if (j < n) {
+            const double f = s * C[a + j * ldc];
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) P.a[jj][t] = fma(f, av[u][t], P.a[jj][t]);
+          }
+        }
       }
     }
   }
@@ -246,6 +266,7 @@ __device__ __forceinline__ void panel_axpy(RegPanel<RPL, NW>& P, const double* A
 // G_out = G_in J, sig[j] = ||G_out[:, j]|| and perm[0..n) lists columns by descending sig (ties by
 // index).  TP threads cooperate on one column pair.  Returns the number of sweeps (-1: no convergence
 // within max_sweeps, -2: a non-finite column norm).  G, J, sig, perm may live in LDS or global memory; flag is an LDS int.
+// sig must hold n entries (it also carries the column norms used for the noise floor).
 template <int TP>
 __device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double* sig, int* perm, int* flag,
                           int max_sweeps) {
@@ -258,8 +279,24 @@ __device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double*
   const int npairs = np / 2;
   const int pi = tid / TP, tl = tid % TP;
   const double tol = 2.220446049250313e-16 * sqrt((double)n);
-  int sweeps = -1;
+  // Columns whose squared norm is below (eps ||G||_F)^2 are rounding noise relative to the matrix: they
+  // are not rotated (their singular values stay below eps ||G||_F either way), which keeps noise-level
+  // columns from being stirred sweep after sweep by the rotations of the large ones.
+  __shared__ double jac_floor;
+  for (int c = tid; c < n; c += nt) {
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) { const double g = G[i + c * ldg]; acc = fma(g, g, acc); }
+    sig[c] = acc;
+  }
   __syncthreads();
+  if (tid == 0) {
+    double fro2 = 0.0;
+    for (int c = 0; c < n; ++c) fro2 += sig[c];
+    jac_floor = 4.930380657631324e-32 * fro2;   // eps^2 ||G||_F^2
+  }
+  __syncthreads();
+  const double floor2 = jac_floor;
+  int sweeps = -1;
   for (int s = 0; s < max_sweeps && n > 1; ++s) {
     if (tid == 0) *flag = 0;
     __syncthreads();
@@ -276,7 +313,7 @@ __device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double*
             al += gp * gp; be += gq * gq; ga += gp * gq;
           }
           al = group_sum<TP>(al); be = group_sum<TP>(be); ga = group_sum<TP>(ga);
-          if (ga != 0.0 && al > 0.0 && be > 0.0 && fabs(ga) > tol * sqrt(al) * sqrt(be)) {
+          if (ga != 0.0 && al > floor2 && be > floor2 && fabs(ga) > tol * sqrt(al) * sqrt(be)) {
             const double zeta = (be - al) / (2.0 * ga);
             const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
             const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;

@@ -69,6 +69,14 @@ struct tlr_ctx {
   std::vector<Slot> slots;
   SlotPtrs* d_slots = nullptr;
   int64_t scr_stride = 0;
+  double* scr_pool = nullptr;          // recompression scratch, one scr_stride per resident work group
+  double* scr_cls[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};   // the part of each class
+  hipStream_t cls_stream[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};   // [0] = stream
+  hipEvent_t ev_list = nullptr, ev_cls[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};
+  int2* wl = nullptr;                  // recompression work lists (RCLASSES x wl_cap)
+  int* wl_count = nullptr;             // nB x RCLASSES list lengths
+  int64_t wl_cap = 0;
+  int recomp_grid[RCLASSES] = {0, 0, 0, 0};   // persistent grid of each recompression class
   int* d_retry = nullptr;              // per slot: tiles redone with the wider sketch
   double* d_thr = nullptr;             // per slot (thr_rel, thr_abs) of the current batch
   double* h_thr = nullptr;             // pinned staging of d_thr
@@ -94,6 +102,11 @@ struct tlr_ctx {
     if (h_replay) hipHostFree(h_replay);
     for (hipEvent_t e : ev_pool) hipEventDestroy(e);
     for (void* p : allocs) hipFree(p);
+    if (ev_list) hipEventDestroy(ev_list);
+    for (int q = 1; q < RCLASSES; ++q) {
+      if (ev_cls[q]) hipEventDestroy(ev_cls[q]);
+      if (cls_stream[q]) hipStreamDestroy(cls_stream[q]);
+    }
     if (stream) hipStreamDestroy(stream);
   }
   hipEvent_t next_event() {
@@ -127,6 +140,10 @@ struct tlr_ctx {
     a.n = n; a.b = b; a.nB = nB; a.npan = npan; a.rank_cap = RANK_CAP;
     a.fstride = (int64_t)b * RANK_CAP;
     a.scr_stride = scr_stride;
+    a.scr_pool = scr_pool;
+    a.wl = wl;
+    a.wl_count = wl_count;
+    a.wl_cap = wl_cap;
     a.rule_fixed = 0;
     a.thr = d_thr;
     a.replay = d_replay;
@@ -215,10 +232,38 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
   ACHK(c->alloc(&c->bufR, (size_t)nt * c->kcap * c->kcap));
   ACHK(c->alloc(&c->part, (size_t)nt * nbb * nbb * 2));
   ACHK(c->alloc(&c->need, (size_t)nt));
-  // recompression scratch per pair: Q2v (b x 64), the right blocks of R_u, R_v (128 x 64 each) and the
-  // Jacobi core and rotations (128 x 128 each) when they do not fit in LDS
+  // recompression: a persistent grid per width class sized to the resident capacity of the device; each
+  // resident work group owns scratch for Q2v (b x 64), the right blocks of R_u, R_v (128 x 64 each) and
+  // the Jacobi core and rotations (128 x 128 each) when they do not fit in LDS
   c->scr_stride = (int64_t)b * 64 + 2 * WCAP * 64 + 2 * WCAP * WCAP;
-  const int64_t npm = c->npairs_max > 0 ? c->npairs_max : 1;
+  {
+    int dev = 0, cus = 0, per_cu[RCLASSES];
+    ACHK(hipGetDevice(&dev));
+    ACHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    ACHK(recompress_occupancy(b, per_cu));
+    int64_t gsum = 0;
+    for (int q = 0; q < RCLASSES; ++q) {
+      c->recomp_grid[q] = (per_cu[q] > 0 ? per_cu[q] : 1) * (cus > 0 ? cus : 1);
+      gsum += c->recomp_grid[q];
+    }
+    // the classes of one step run concurrently (one stream each), so each has its own scratch
+    ACHK(c->alloc(&c->scr_pool, (size_t)gsum * c->scr_stride));
+    int64_t off = 0;
+    for (int q = 0; q < RCLASSES; ++q) {
+      c->scr_cls[q] = c->scr_pool + off * c->scr_stride;
+      off += c->recomp_grid[q];
+    }
+    c->cls_stream[0] = c->stream;
+    for (int q = 1; q < RCLASSES; ++q) {
+      ACHK(hipStreamCreateWithFlags(&c->cls_stream[q], hipStreamNonBlocking));
+      ACHK(hipEventCreateWithFlags(&c->ev_cls[q], hipEventDisableTiming));
+    }
+    ACHK(hipEventCreateWithFlags(&c->ev_list, hipEventDisableTiming));
+    const int64_t npm = c->npairs_max > 0 ? c->npairs_max : 1;
+    c->wl_cap = npm * max_batch;
+    ACHK(c->alloc(&c->wl, (size_t)RCLASSES * c->wl_cap));
+    ACHK(c->alloc(&c->wl_count, (size_t)RCLASSES * c->nB));
+  }
   c->slots.resize(max_batch);
   for (int s = 0; s < max_batch; ++s) {
     SlotPtrs& p = c->slots[s].p;
@@ -234,7 +279,6 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
     ACHK(c->alloc(&p.logdet_part, (size_t)c->nB));
     ACHK(c->alloc(&p.status, (size_t)ST_COUNT));
     ACHK(c->alloc(&p.result, (size_t)4));
-    ACHK(c->alloc(&p.scratch, (size_t)npm * c->scr_stride));
   }
   // rank logs: offsets of each step's recompressions, one log per slot and max_batch replay records
   {
@@ -348,6 +392,26 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
 }  // extern "C"
 
 // One pass over m theta; replay[s] >= 0 makes slot s reuse the ranks of record replay[s] (frozen ranks).
+// Recompressions of step k: the work lists on the main stream, then the four width classes at once, the
+// wide ones on their own streams (their few large pairs take long; the narrow class fills the device
+// meanwhile), joined back into the main stream.
+static hipError_t launch_recompress_step(tlr_ctx* c, const SlotArgs& sa, int k, int nslot) {
+  if (c->nB - k - 1 < 2) return hipSuccess;
+  hipError_t e = launch_recomp_list(sa, k, nslot, c->stream);
+  if (e != hipSuccess) return e;
+  if ((e = hipEventRecord(c->ev_list, c->stream)) != hipSuccess) return e;
+  for (int q = 1; q < RCLASSES; ++q) {
+    if ((e = hipStreamWaitEvent(c->cls_stream[q], c->ev_list, 0)) != hipSuccess) return e;
+    if ((e = launch_recomp_class(sa, k, q, c->recomp_grid[q], c->scr_cls[q], c->cls_stream[q])) != hipSuccess)
+      return e;
+    if ((e = hipEventRecord(c->ev_cls[q], c->cls_stream[q])) != hipSuccess) return e;
+  }
+  if ((e = launch_recomp_class(sa, k, 0, c->recomp_grid[0], c->scr_cls[0], c->stream)) != hipSuccess) return e;
+  for (int q = 1; q < RCLASSES; ++q)
+    if ((e = hipStreamWaitEvent(c->stream, c->ev_cls[q], 0)) != hipSuccess) return e;
+  return hipSuccess;
+}
+
 static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
                      const int* replay, double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma,
                      int32_t* ranks_factor) {
@@ -364,6 +428,7 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
   c->ev_used = 0;
   c->timed.clear();
   HIPCHK(launch_fill_int(c->d_retry, c->max_batch, 0, st));
+  HIPCHK(launch_fill_int(c->wl_count, (int64_t)RCLASSES * c->nB, 0, st));
   // per-slot thresholds of the recompression (the adaptive budget depends on theta); the previous call
   // ended with a synchronisation, so the pinned staging buffer is free
   const double bmax_ = (double)(c->n < c->b ? c->n : c->b);
@@ -397,7 +462,7 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
     HIPCHK(launch_potrf(sa, k, m, st));
     HIPCHK(launch_trsm(sa, k, m, st));
     HIPCHK(launch_diag_update(sa, k, m, st));
-    HIPCHK(launch_recompress(sa, k, m, st));
+    HIPCHK(launch_recompress_step(c, sa, k, m));
   }
   HIPCHK(launch_final(sa, m, st));
   HIPCHK(hipStreamSynchronize(st));

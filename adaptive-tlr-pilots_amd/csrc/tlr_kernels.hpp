@@ -10,6 +10,9 @@ namespace tlr {
 // (rank or core out of range: the result is unusable and reported as an error)
 enum { ST_RANKCAP = 0, ST_FAILED = 1, ST_GUARD = 2, ST_COUNT = 4 };
 
+// width classes of the recompression, by r_jk: 1-8, 9-16, 17-32, 33-64
+constexpr int RCLASSES = 4;
+
 // Device buffers of one slot (one theta being evaluated).  Layouts (all FP64 column-major):
 //   D      nB tiles of b x b: diagonal tiles, factored in place into L_kk (lower)
 //   Dinv   nB x npan blocks of 32 x 32: inverses of the diagonal blocks of L_kk
@@ -17,7 +20,6 @@ enum { ST_RANKCAP = 0, ST_FAILED = 1, ST_GUARD = 2, ST_COUNT = 4 };
 //   rank   ntile ints (current rank), rank0 the ranks after the initial compression
 //   Wd, Xd nB x (rank_cap^2) and nB x (b x rank_cap): W_i = V_ik^T V_ik and U_ik W_i of step k
 //   rhs    N: y on entry, z = L^{-1} y on exit
-//   scratch per-pair workspace of the recompression
 struct SlotPtrs {
   double* D;
   double* Dinv;
@@ -31,7 +33,6 @@ struct SlotPtrs {
   double* logdet_part;
   int* status;
   double* result;   // [loglik, logdet, quad]
-  double* scratch;
   int* rlog;          // rank chosen by every recompression, at rlog_off[k] + pair index
 };
 
@@ -39,7 +40,11 @@ struct SlotArgs {
   int64_t n;
   int b, nB, npan, rank_cap;
   int64_t fstride;      // b * rank_cap
-  int64_t scr_stride;   // doubles of scratch per recompression pair
+  int64_t scr_stride;   // doubles of recompression scratch per resident work group
+  double* scr_pool;     // scratch of the running recompression class (shared by all slots)
+  int2* wl;             // recompression work lists: RCLASSES lists of wl_cap (pair, slot) entries
+  int* wl_count;        // nB x RCLASSES entry counts (zeroed at the start of every evaluation)
+  int64_t wl_cap;
   int rule_fixed;       // 1: fixed relative tolerance, 0: adaptive absolute budget
   const double* thr;    // per slot: (thr_rel, thr_abs)
   const int* replay;    // per slot: -1, or the record whose ranks this evaluation reuses (frozen ranks)
@@ -74,7 +79,10 @@ hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t 
 hipError_t launch_potrf(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_trsm(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t s);
-hipError_t launch_recompress(const SlotArgs& sa, int k, int nslot, hipStream_t s);
+hipError_t launch_recomp_list(const SlotArgs& sa, int k, int nslot, hipStream_t s);
+hipError_t launch_recomp_class(const SlotArgs& sa, int k, int cls, int grid, double* scr, hipStream_t s);
+// resident work groups per CU of each recompression class (b selects the row-panel variant)
+hipError_t recompress_occupancy(int b, int blocks_per_cu[RCLASSES]);
 hipError_t launch_final(const SlotArgs& sa, int nslot, hipStream_t s);
 
 // solve.hip

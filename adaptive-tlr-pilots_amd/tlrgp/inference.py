@@ -125,10 +125,10 @@ def fit_mle(S, y, bounds, rule, value, x0, maxiter=300):
 
 
 def fit_mle_grad(S, y, bounds, rule, value, x0, maxiter=100, h=1e-5):
-    """L-BFGS-B in log theta inside the box LO..HI using log L and its frozen-rank gradient.  The result
-    is a point where the frozen-rank gradient vanishes; with adaptive ranks that need not be the optimum a
-    derivative-free search finds, because rank changes between nearby theta shift log L by small jumps that
-    the gradient does not see."""
+    """L-BFGS-B in log theta inside the box LO..HI using log L and its frozen-rank gradient.  With the
+    adaptive rule the objective re-chooses ranks at every theta (budget proportional to sigma2 * eta); the
+    rank changes add a drift to log L that the frozen-rank gradient does not see, so the line search can
+    stop short of the optimum a derivative-free search finds (on the same likelihood ridge)."""
     from scipy import optimize
     from .likelihood import tlr_loglik_grad
     trace = dict(evals=0, failures=0)
