@@ -11,18 +11,25 @@ It restates, in NumPy/SciPy, the semantics of the reference scripts
 * ``codes/17_tlr_cholesky_pilot.py`` -- per-tile truncated SVD, QR+SVD recompression, the right-looking
   TLR Cholesky, the tile forward/backward solves and the TLR log-likelihood.
 
-The storage-precision ``mixed`` rule of 17 is not part of the hot path and is not restated.  In
-addition to the reference's ``info`` dictionary the factorization reports the per-tile ranks (initial
-and final, as dense lower-triangular nB x nB int arrays), the log-determinant and the quadratic form,
-which the reference computes but never writes.
+The storage-precision ``mixed`` rule of 17 (adaptive ranks, then every factor pair stored in the lowest
+of FP8-E4M3 / FP16 / FP32 / FP64 whose rounding error fits the budget left after truncation, arithmetic
+in FP64) is restated in ``round_to`` / ``stored`` / ``round_pair``.  In addition to the reference's
+``info`` dictionary the factorization reports the per-tile ranks (initial and final, as dense
+lower-triangular nB x nB int arrays), the log-determinant and the quadratic form, which the reference
+computes but never writes.
 
 Pinning (see ``tests/test_oracle_golden.py``): the dense log-likelihood, the TLR log-likelihood, stored
 values, mean/max factor rank, completion flags and failing tile of every rule are checked against the
 reference's committed result JSONs on the reference's own committed inputs; Morton order, tile bounds
 and the kernel against closed forms; the rank rule against matrices with known singular values;
 ``recompress`` against a direct SVD of the product; the TLR factor against the dense Cholesky at a
-vanishing tolerance.  The analytic dense gradient is pinned by central finite differences of
-``loglik_dense``.
+vanishing tolerance; the rounding of the ``mixed`` rule against closed forms (representable values,
+ties to even, saturation, IEEE half/single conversions) and its precision choice by brute force.  The
+``mixed`` factorization itself is pinned only statistically (log L within 0.5, tile counts per precision
+within 10, stored values within 0.5 % of the reference's run at N = 10k): its rank and precision decisions
+feed each other, so implementation details of the FP16/FP8 conversion move single decisions and with
+them the rest of the factorization.  The analytic dense gradient is pinned by central finite
+differences of ``loglik_dense``.
 """
 import numpy as np
 from scipy.linalg import cho_factor, cho_solve, cholesky, solve_triangular, LinAlgError
@@ -239,13 +246,66 @@ def recompress(U, V, rule, thr_rel, thr_abs):
 
 
 def rule_thresholds(rule, value, theta, n, b):
-    """(thr_rel, thr_abs) of 17:149-150: fixed -> eps; adaptive -> tau * b * sigma2 * eta / N (b = max tile)."""
+    """(thr_rel, thr_abs) of 17:149-150: fixed -> eps; adaptive and mixed -> tau * b * sigma2 * eta / N
+    (b = max tile)."""
     sigma2, _, eta = theta
     if rule == "fixed":
         return value, None
-    if rule == "adaptive":
+    if rule in ("adaptive", "mixed"):
         return None, value * b * sigma2 * eta / n
     raise ValueError(f"unknown rule {rule!r}")
+
+
+# ----------------------------------------------------------------------------- storage precision (mixed)
+PRECISIONS = (8, 16, 32, 64)
+
+
+def round_to(x, p):
+    """Round to the values of a p-bit format and back to float64 (17:41-59): FP32 and FP16 as NumPy's
+    conversions; FP8 as 17 defines it: exponent floor(log2|x|) clipped to [-9, 8], 3 fraction bits with
+    round-half-even, magnitudes capped at 448 (no subnormal range below the clip)."""
+    if p == 64:
+        return x
+    if p == 32:
+        return x.astype(np.float32).astype(np.float64)
+    if p == 16:
+        return x.astype(np.float16).astype(np.float64)
+    a = np.abs(x)
+    out = np.zeros_like(x)
+    nz = a > 0
+    e = np.clip(np.floor(np.log2(a[nz])), -9, 8)
+    v = np.round(a[nz] / 2.0 ** e * 8.0) / 8.0 * 2.0 ** e
+    v[v > 448.0] = 448.0
+    out[nz] = np.sign(x[nz]) * v
+    return out
+
+
+def stored(U, p):
+    """Stored values of U in format p (17:62-69): FP8/FP16 scaled so that max|U| maps to half the
+    format's largest value (224, 32752), FP32 unscaled."""
+    if p == 64 or U.size == 0:
+        return U
+    sc = np.abs(U).max()
+    if sc == 0:
+        return U
+    cap = {8: 448.0 * 0.5, 16: 65504.0 * 0.5, 32: 1.0}[p]
+    scale = cap / sc if p < 32 else 1.0
+    return round_to(U * scale, p) / scale
+
+
+def round_pair(U, V, budget):
+    """(U', V', p): the lowest p whose rounding error ||U V^T - U' V'^T||_F is within budget
+    (17:72-82); rank 0 stays FP64."""
+    if U.shape[1] == 0:
+        return U, V, 64
+    T = U @ V.T
+    for p in PRECISIONS:
+        if p == 64:
+            return U, V, 64
+        Ur, Vr = stored(U, p), stored(V, p)
+        if np.linalg.norm(T - Ur @ Vr.T) <= budget:
+            return Ur, Vr, p
+    return U, V, 64
 
 
 # ----------------------------------------------------------------------------- TLR Cholesky
@@ -260,6 +320,10 @@ def tlr_cholesky(S, theta, bounds, rule, value):
     b = max(e - s for s, e in bounds)
     sigma2, r, eta = theta
     thr_rel, thr_abs = rule_thresholds(rule, value, theta, n, b)
+    mixed = rule == "mixed"   # adaptive ranks + rounded storage (17:146-147, 163-164, 183-184, 203-204)
+    if mixed:
+        rule = "adaptive"
+    prec = np.zeros((nB, nB), np.int64)
     A_diag, A_off = [], {}
     ranks0 = np.zeros((nB, nB), np.int64)
     stored0 = 0
@@ -271,7 +335,9 @@ def tlr_cholesky(S, theta, bounds, rule, value):
         stored0 += (i1 - i0) ** 2
         for j, (j0, j1) in enumerate(bounds[:i]):
             T = sigma2 * matern32(dist(S[i0:i1], S[j0:j1]), r)
-            U, V, _ = truncate_svd(T, rule, thr_rel, thr_abs)
+            U, V, tail = truncate_svd(T, rule, thr_rel, thr_abs)
+            if mixed:
+                U, V, _ = round_pair(U, V, thr_abs - tail)
             A_off[(i, j)] = (U, V)
             ranks0[i, j] = U.shape[1]
             stored0 += 2 * U.shape[1] * max(i1 - i0, j1 - j0)
@@ -290,6 +356,8 @@ def tlr_cholesky(S, theta, bounds, rule, value):
             U, V = A_off.pop((i, k))
             if U.shape[1]:
                 V = solve_triangular(Lkk, V, lower=True, check_finite=False)
+            if mixed:
+                U, V, prec[i, k] = round_pair(U, V, thr_abs)
             L_off[(i, k)] = (U, V)
         # trailing update A_ij -= L_ik L_jk^T for k < j <= i (17:187-205)
         for i in range(k + 1, nB):
@@ -305,8 +373,10 @@ def tlr_cholesky(S, theta, bounds, rule, value):
                     A_diag[i] -= (Ui @ W) @ Ui.T
                 else:
                     Uij, Vij = A_off[(i, j)]
-                    Ur, Vr, _ = recompress(np.hstack([Uij, -(Ui @ W)]), np.hstack([Vij, Uj]),
-                                           rule, thr_rel, thr_abs)
+                    Ur, Vr, tail = recompress(np.hstack([Uij, -(Ui @ W)]), np.hstack([Vij, Uj]),
+                                              rule, thr_rel, thr_abs)
+                    if mixed:
+                        Ur, Vr, _ = round_pair(Ur, Vr, thr_abs - tail)
                     A_off[(i, j)] = (Ur, Vr)
     ranksF = np.zeros((nB, nB), np.int64)
     for (i, j), (U, _) in L_off.items():
@@ -316,6 +386,13 @@ def tlr_cholesky(S, theta, bounds, rule, value):
     info.update(completed=True, stored_factor=int(stored),
                 mean_rank_factor=float(np.mean(rk)) if rk else 0.0,
                 max_rank_factor=int(max(rk)) if rk else 0, ranks_factor=ranksF)
+    if mixed:   # 17:210-214
+        byt = sum(d.shape[0] ** 2 * 8 for d in L_diag) + \
+            sum(2 * u.shape[1] * u.shape[0] * int(prec[i, j]) // 8 for (i, j), (u, _) in L_off.items())
+        info.update(stored_factor_bytes=int(byt), dense_factor_bytes=int(n * (n + 1) // 2 * 8),
+                    factor_bytes_frac=byt / (n * (n + 1) / 2 * 8),
+                    precision_mix_factor_tiles={p: int(sum(1 for k in L_off if prec[k] == p)) for p in PRECISIONS},
+                    precision_factor=prec)
     return L_diag, L_off, info
 
 

@@ -9,7 +9,8 @@ Reads (text only, no code is run) the reference's ``results/`` folder:
 * the paired result JSONs: dense and TLR log-likelihoods, stored values, mean/max factor ranks,
   completion flags and failing tiles per rule, the TLR log-determinant of ``kl_tlr.json``, the dense
   quadratic form of ``bound_terms_n{N}.json``, the Nelder-Mead MLE results (theta_hat, objective,
-  evaluation and failure counts) of the N = 10k and 20k runs.  Written to ``goldens.json``;
+  evaluation and failure counts) of the N = 10k and 20k runs, the ``mixed`` storage-precision rule's
+  results at N = 10k and 20k (``tlr_cholesky_mixed/``).  Written to ``goldens.json``;
 * ``mcmc_tlrchol/<rule>/c<k>/``: the first 1200 draws of the reference's random-walk chains at N = 5000
   (adaptive 0.9 chains 1-4, fixed 1e-4 chains 1 and 4) with their settings, as ``mcmc_n5000_heads.npz``
   and the ``mcmc_n5000`` entry of ``goldens.json``.
@@ -65,6 +66,14 @@ def main(ref="/root/reference"):
     for n in (10000, 40000):
         bt = _load_json(ref, f"bound_terms/bound_terms_n{n}.json")
         g["bound_terms"][str(n)] = dict(dense_loglik=bt["dense_loglik"], z2=bt["z2"])
+    g["mixed"] = {}   # the storage-precision rule mixed:0.9 (adaptive ranks + FP8/16/32/64 factor storage)
+    for n in (10000, 20000):
+        d = _load_json(ref, f"tlr_cholesky_mixed/rules_only/tlr_cholesky_n{n}.json") if n == 10000 else \
+            _load_json(ref, f"tlr_cholesky_mixed/tlr_cholesky_n{n}.json")
+        r = next(x for x in d["rules"] if x["rule"] == "mixed")
+        g["mixed"][str(n)] = {k: r[k] for k in ("value", "completed", "stored_sigma", "stored_factor",
+                                               "stored_factor_bytes", "precision_mix_factor_tiles",
+                                               "mean_rank_factor", "max_rank_factor", "loglik", "loglik_err")}
     g["mle"] = {}
     for n in (10000, 20000):
         d = _load_json(ref, f"tlr_cholesky/tlr_cholesky_n{n}.json")
