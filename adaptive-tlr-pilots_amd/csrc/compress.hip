@@ -14,6 +14,7 @@
 // `pmin` spare sketch columns is redone with a wider sketch.
 #include "common.hpp"
 #include "panel.hpp"
+#include "round.hpp"
 #include "tlr_kernels.hpp"
 
 namespace tlr {
@@ -44,8 +45,8 @@ __global__ void k_gen_diag(const double* __restrict__ sx, const double* __restri
 // out_t (b x k) = op(T_t) X_t for every active tile t.  op = 0: T (rows: sites of tile i, inner index:
 // sites of tile j); op = 1: T^T (the roles of the two site sets swap).  X_t has leading dimension b;
 // xstride = 0 shares one X (the Gaussian test matrix) across tiles.  Work group: 256 threads computing a
-// 256-row x 64-column block as 8 x 8 register blocks; the inner index runs in chunks of 16 whose kernel
-// entries are generated once into LDS.
+// 256-row x 64-column block as 8 x 8 register blocks (rows strided by 32 so that stores coalesce); the
+// inner index runs in chunks of 16 whose kernel entries are generated once into LDS.
 constexpr int MP_ROWS = 256, MP_COLS = 64, MP_QC = 16;
 
 __global__ __launch_bounds__(256) void k_matern_product(const double* __restrict__ sx, const double* __restrict__ sy,
@@ -66,7 +67,9 @@ __global__ __launch_bounds__(256) void k_matern_product(const double* __restrict
   __shared__ double rx[MP_ROWS], ry[MP_ROWS];
   __shared__ double qx[MP_QC], qy[MP_QC];
   const int tid = threadIdx.x;
-  const int tr = tid / 8, tc = tid % 8;    // 32 row groups x 8 column groups
+  // thread (tr, tc) owns rows rb + 32 a + tr (a < 8) and columns cb + 8 tc + c (c < 8): the 32 lanes of a
+  // column group read consecutive LDS words and store 32 consecutive rows of one column (256 bytes)
+  const int tr = tid % 32, tc = tid / 32;
   {
     const int p = rb + tid;
     rx[tid] = (p < mr) ? sx[r0g + p] : 0.0;
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(256) void k_matern_product(const double* __restrict
     for (int qq = 0; qq < MP_QC; ++qq) {
       double tv[8], xv[8];
 #pragma unroll
-      for (int a = 0; a < 8; ++a) tv[a] = Ts[qq][tr * 8 + a];
+      for (int a = 0; a < 8; ++a) tv[a] = Ts[qq][32 * a + tr];
 #pragma unroll
       for (int c = 0; c < 8; ++c) xv[c] = Xs[qq][tc * 8 + c];
 #pragma unroll
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(256) void k_matern_product(const double* __restrict
   double* Ot = out + t * ostride;
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
-    const int p = rb + tr * 8 + a;
+    const int p = rb + 32 * a + tr;
     if (p < b) {
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
@@ -233,14 +236,14 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ sx,
 }
 
 // ------------------------------------------------------------------------------ rank decision
-__global__ __launch_bounds__(256) void k_rank_decide(const double* __restrict__ Rm, int64_t rstride,
+__global__ __launch_bounds__(256) void k_rank_decide(double* __restrict__ Rm, int64_t rstride,
                                                      const double* __restrict__ Qrow, const double* __restrict__ Qcol,
                                                      int64_t qstride, const double* __restrict__ part, int b, int k,
                                                      int kmax, int pmin, int rule_fixed, double thr_rel, double thr_abs,
                                                      double* __restrict__ U, double* __restrict__ V, int64_t fstride,
                                                      int rank_cap, int* __restrict__ rank, int* __restrict__ need,
                                                      int* __restrict__ retry_count, int* __restrict__ status,
-                                                     const int* __restrict__ forced) {
+                                                     const int* __restrict__ forced, int mixed) {
   const int64_t t = blockIdx.x;
   if (!need[t]) return;
   extern __shared__ double lds[];
@@ -252,6 +255,7 @@ __global__ __launch_bounds__(256) void k_rank_decide(const double* __restrict__ 
   __shared__ int flag;
   __shared__ int s_rank;
   __shared__ double s_res2, s_fro2;
+  __shared__ double s_red[8];
   const int tid = threadIdx.x;
   const double* Rt = Rm + t * rstride;
   for (int e = tid; e < k * k; e += blockDim.x) G[e] = Rt[e];
@@ -317,6 +321,12 @@ __global__ __launch_bounds__(256) void k_rank_decide(const double* __restrict__ 
     }
     Ut[p + (int64_t)c * b] = u;
     Vt[p + (int64_t)c * b] = v;
+  }
+  if (mixed) {
+    // storage precision under the budget left after truncation (17:163-164); the core R of this tile,
+    // already copied to LDS, gives the Gram workspace (ranks above sqrt(rstride / 3) stay FP64)
+    __syncthreads();
+    if (3 * r * r <= rstride) round_pair_wg(Ut, b, b, Vt, b, b, r, thr_abs - tail[r], Rm + t * rstride, s_red);
   }
   if (tid == 0) { rank[t] = r; need[t] = 0; }
 }
@@ -400,11 +410,11 @@ size_t rank_decide_lds(int k) {
   return (size_t)(2 * k * k + 2 * k + 1) * sizeof(double) + (size_t)k * sizeof(int);
 }
 
-hipError_t launch_rank_decide(const double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
+hipError_t launch_rank_decide(double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
                               int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
                               double thr_rel, double thr_abs, double* U, double* V, int64_t fstride, int rank_cap,
-                              int* rank, int* need, int* retry_count, int* status, const int* forced, int64_t ntile,
-                              hipStream_t s) {
+                              int* rank, int* need, int* retry_count, int* status, const int* forced, int mixed,
+                              int64_t ntile, hipStream_t s) {
   const size_t lds = rank_decide_lds(k);
   if (lds > 158 * 1024) return hipErrorInvalidValue;
   static bool attr_set = false;
@@ -414,7 +424,7 @@ hipError_t launch_rank_decide(const double* Rm, int64_t rstride, const double* Q
   }
   hipLaunchKernelGGL(k_rank_decide, dim3((unsigned)ntile), dim3(256), lds, s, Rm, rstride, Qrow, Qcol, qstride, part,
                      b, k, kmax, pmin, rule_fixed, thr_rel, thr_abs, U, V, fstride, rank_cap, rank, need, retry_count,
-                     status, forced);
+                     status, forced, mixed);
   return hipGetLastError();
 }
 

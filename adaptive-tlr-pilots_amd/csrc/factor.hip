@@ -11,6 +11,7 @@
 // left-looking solve.  Updates with r_ik = 0 or r_jk = 0 are skipped, as in the reference.
 #include "common.hpp"
 #include "panel.hpp"
+#include "round.hpp"
 #include "tlr_kernels.hpp"
 
 namespace tlr {
@@ -259,9 +260,17 @@ __global__ __launch_bounds__(256) void k_trsm(SlotArgs sa, int k) {
   // W_i = V^T V and X_i = U W
   double* Wi = P.Wd + (int64_t)(k + item) * rc * rc;
   double* Xi = P.Xd + (int64_t)(k + item) * b * rc;
-  const double* U = P.U + tik * sa.fstride;
+  double* U = P.U + tik * sa.fstride;
   __shared__ double Wsh[64 * 64];
   const int w = wave_id(), l = lane_id();
+  if (sa.mixed) {
+    // mixed rule: storage precision of L_ik under the full budget (17:183-184), before W_i and X_i use it;
+    // X_i's slot is the Gram workspace (b x rank_cap >= 3 r^2)
+    __shared__ double red[8];
+    const int bi = (int)min<int64_t>(b, sa.n - (int64_t)(k + item) * b);
+    const int pr = 3 * r * r <= b * rc ? round_pair_wg(U, b, bi, X, b, bk, r, sa.thr[2 * slot + 1], Xi, red) : 64;
+    if (tid == 0) P.prec[tik] = pr;
+  }
   for (int e = w; e < r * r; e += 4) {
     const int a = e % r, c = e / r;
     if (a > c) continue;
@@ -456,6 +465,7 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
   __shared__ double sig[L::WMAX];
   __shared__ int perm[L::WMAX];
   __shared__ int flag, s_rank;
+  __shared__ double s_tail;
   const int tid = threadIdx.x, NT = 64 * NW, wv = wave_id(), ln = lane_id();
   const int64_t fs = sa.fstride;
   double* Uij = P.U + tij * fs;
@@ -536,6 +546,7 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
     if (r > sa.rank_cap) { atomicMax(&P.status[ST_RANKCAP], 1); r = sa.rank_cap; }
     P.rlog[lo] = r;
     s_rank = r;
+    s_tail = sqrt(acc);
   }
   __syncthreads();
   const int rn = s_rank;
@@ -584,6 +595,12 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
       }
       __syncthreads();
     }
+  }
+  if (sa.mixed) {
+    // mixed rule: storage precision under the budget left after truncation (17:203-204); the work
+    // group's scratch (Q2v is dead by now) holds the Gram matrices, sig[] the reductions
+    const int bi = (int)min<int64_t>(b, sa.n - (int64_t)i * b), bj = (int)min<int64_t>(b, sa.n - (int64_t)j * b);
+    round_pair_wg(Uij, b, bi, Vij, b, bj, rn, thr_abs - s_tail, scr, sig);
   }
   if (tid == 0) P.rank[tij] = rn;
 }

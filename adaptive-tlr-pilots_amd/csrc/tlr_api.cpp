@@ -97,17 +97,17 @@ struct tlr_ctx {
   double timed_ms = 0.0, timed_flops = 0.0;
   int64_t timed_launches = 0;
   ~tlr_ctx() {
-    if (pred_buf) hipFree(pred_buf);
-    if (h_thr) hipHostFree(h_thr);
-    if (h_replay) hipHostFree(h_replay);
-    for (hipEvent_t e : ev_pool) hipEventDestroy(e);
-    for (void* p : allocs) hipFree(p);
-    if (ev_list) hipEventDestroy(ev_list);
+    if (pred_buf) (void)hipFree(pred_buf);
+    if (h_thr) (void)hipHostFree(h_thr);
+    if (h_replay) (void)hipHostFree(h_replay);
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    for (void* p : allocs) (void)hipFree(p);
+    if (ev_list) (void)hipEventDestroy(ev_list);
     for (int q = 1; q < RCLASSES; ++q) {
-      if (ev_cls[q]) hipEventDestroy(ev_cls[q]);
-      if (cls_stream[q]) hipStreamDestroy(cls_stream[q]);
+      if (ev_cls[q]) (void)hipEventDestroy(ev_cls[q]);
+      if (cls_stream[q]) (void)hipStreamDestroy(cls_stream[q]);
     }
-    if (stream) hipStreamDestroy(stream);
+    if (stream) (void)hipStreamDestroy(stream);
   }
   hipEvent_t next_event() {
     if (ev_used == ev_pool.size()) {
@@ -145,6 +145,7 @@ struct tlr_ctx {
     a.wl_count = wl_count;
     a.wl_cap = wl_cap;
     a.rule_fixed = 0;
+    a.mixed = 0;
     a.thr = d_thr;
     a.replay = d_replay;
     a.rep_rlog = rep_rlog;
@@ -279,6 +280,7 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
     ACHK(c->alloc(&p.logdet_part, (size_t)c->nB));
     ACHK(c->alloc(&p.status, (size_t)ST_COUNT));
     ACHK(c->alloc(&p.result, (size_t)4));
+    ACHK(c->alloc(&p.prec, (size_t)nt));
   }
   // rank logs: offsets of each step's recompressions, one log per slot and max_batch replay records
   {
@@ -330,7 +332,7 @@ int tlr_ctx_set_y(tlr_ctx* ctx, const double* y) {
 }
 
 static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternParams& mp, int rule_fixed,
-                         double thr_rel, double thr_abs, const int* forced) {
+                         double thr_rel, double thr_abs, const int* forced, int mixed) {
   hipStream_t s = c->stream;
   const int b = c->b;
   const int64_t nt = c->ntile;
@@ -383,7 +385,7 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
     const double* Qcol = (last_op == 0) ? Qprev : M;
     HIPCHK(launch_rank_decide(c->bufR, rs, Qrow, Qcol, ms, c->part, b, k, kmax, PMIN, rule_fixed, thr_rel, thr_abs,
                               p.U, p.V, (int64_t)b * RANK_CAP, RANK_CAP, p.rank, c->need, c->d_retry + slot,
-                              p.status, forced, nt, s));
+                              p.status, forced, mixed, nt, s));
   }
   HIPCHK(launch_copy_int(p.rank0, p.rank, nt, s));
   return TLR_OK;
@@ -391,12 +393,13 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
 
 }  // extern "C"
 
-// One pass over m theta; replay[s] >= 0 makes slot s reuse the ranks of record replay[s] (frozen ranks).
 // Recompressions of step k: the work lists on the main stream, then the four width classes at once, the
 // wide ones on their own streams (their few large pairs take long; the narrow class fills the device
 // meanwhile), joined back into the main stream.
 static hipError_t launch_recompress_step(tlr_ctx* c, const SlotArgs& sa, int k, int nslot) {
   if (c->nB - k - 1 < 2) return hipSuccess;
+  TimedLaunch tt{nullptr, nullptr, -1, k, 0.0};
+  const bool on = c->tic(5, tt);
   hipError_t e = launch_recomp_list(sa, k, nslot, c->stream);
   if (e != hipSuccess) return e;
   if ((e = hipEventRecord(c->ev_list, c->stream)) != hipSuccess) return e;
@@ -409,14 +412,47 @@ static hipError_t launch_recompress_step(tlr_ctx* c, const SlotArgs& sa, int k, 
   if ((e = launch_recomp_class(sa, k, 0, c->recomp_grid[0], c->scr_cls[0], c->stream)) != hipSuccess) return e;
   for (int q = 1; q < RCLASSES; ++q)
     if ((e = hipStreamWaitEvent(c->stream, c->ev_cls[q], 0)) != hipSuccess) return e;
+  c->toc(on, tt);
   return hipSuccess;
 }
 
+// Algorithmic flops of every recompression of one evaluation, replayed on the host from the initial ranks
+// and the rank log: pair (i, j) of step k with r_ik, r_jk > 0 costs 4 b r_ik r_jk (W and U_ik W) +
+// 8 b w^2 (QR of both concatenations) + 22 w^3 (core SVD) + 4 b w r' (rebuilt factors), w = r_ij + r_jk,
+// r' the rank it was truncated to.  Steps from failed_at on are skipped (they did not run).
+static double recompress_flops(const tlr_ctx* c, const std::vector<int>& rank0, const std::vector<int>& rfin,
+                               const std::vector<int>& rlog, int failed_at) {
+  std::vector<int> cur(rank0);
+  const double b = (double)c->b;
+  double fl = 0.0;
+  int64_t off = 0;
+  const int kend = failed_at >= 0 && failed_at < c->nB ? failed_at : c->nB;
+  for (int k = 0; k < kend; ++k) {
+    const int mk = c->nB - k - 1;
+    int64_t q = 0;
+    for (int a = 1; a < mk; ++a)
+      for (int bb = 0; bb < a; ++bb, ++q) {
+        const int i = a + k + 1, j = bb + k + 1;
+        const int rik = rfin[tri_index(i, k)], rjk = rfin[tri_index(j, k)];
+        if (rik == 0 || rjk == 0) continue;
+        const int64_t tij = tri_index(i, j);
+        const double w = (double)(cur[tij] + rjk), rn = (double)rlog[off + q];
+        fl += 4.0 * b * rik * rjk + 8.0 * b * w * w + 22.0 * w * w * w + 4.0 * b * w * rn;
+        cur[tij] = rlog[off + q];
+      }
+    off += (int64_t)mk * (mk - 1) / 2;
+  }
+  return fl;
+}
+
+// One pass over m theta; replay[s] >= 0 makes slot s reuse the ranks of record replay[s] (frozen ranks).
 static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
                      const int* replay, double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma,
                      int32_t* ranks_factor) {
   if (!c || !thetas || !out || m < 1 || m > c->max_batch) return set_err(TLR_ERR_ARG, "invalid argument");
-  if (rule != TLR_RULE_FIXED && rule != TLR_RULE_ADAPTIVE) return set_err(TLR_ERR_ARG, "unknown rule");
+  if (rule != TLR_RULE_FIXED && rule != TLR_RULE_ADAPTIVE && rule != TLR_RULE_MIXED)
+    return set_err(TLR_ERR_ARG, "unknown rule");
+  if (rule == TLR_RULE_MIXED && replay) return set_err(TLR_ERR_ARG, "frozen-rank replay is not available for the mixed rule");
   if (precision != TLR_PREC_FP64) return set_err(TLR_ERR_ARG, "unsupported precision");
   for (int s = 0; s < m; ++s) {
     const double* th = thetas + 3 * s;
@@ -425,6 +461,7 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
   if (!(value > 0.0)) return set_err(TLR_ERR_ARG, "rule value must be positive");
   hipStream_t st = c->stream;
   const int rule_fixed = rule == TLR_RULE_FIXED;
+  const int mixed = rule == TLR_RULE_MIXED;
   c->ev_used = 0;
   c->timed.clear();
   HIPCHK(launch_fill_int(c->d_retry, c->max_batch, 0, st));
@@ -453,11 +490,13 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
     HIPCHK(launch_fill_int(p.status + ST_FAILED, 1, 0x7fffffff, st));
     HIPCHK(hipMemcpyAsync(p.rhs, c->y, c->n * sizeof(double), hipMemcpyDeviceToDevice, st));
     const int* forced = (replay && replay[s] >= 0) ? c->rep_rank0 + (int64_t)replay[s] * c->ntile : nullptr;
-    int rc = compress_slot(c, s, p, mp, rule_fixed, value, thr_abs, forced);
+    if (mixed) HIPCHK(launch_fill_int(p.prec, c->ntile, 64, st));
+    int rc = compress_slot(c, s, p, mp, rule_fixed, value, thr_abs, forced, mixed);
     if (rc != TLR_OK) return rc;
   }
   SlotArgs sa = c->args();
   sa.rule_fixed = rule_fixed;
+  sa.mixed = mixed;
   for (int k = 0; k < c->nB; ++k) {
     HIPCHK(launch_potrf(sa, k, m, st));
     HIPCHK(launch_trsm(sa, k, m, st));
@@ -472,12 +511,27 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
     for (const TimedLaunch& t : c->timed) {
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
-      const double tiles = t.round == 0 ? (double)c->ntile : (double)retry[t.slot];
       c->timed_ms += ms;
-      c->timed_flops += tiles * t.flops_per_tile;
       c->timed_launches += 1;
+      if (c->timing_kernel != 5) {
+        const double tiles = t.round == 0 ? (double)c->ntile : (double)retry[t.slot];
+        c->timed_flops += tiles * t.flops_per_tile;
+      }
     }
     c->timed.clear();
+    if (c->timing_kernel == 5 && c->ntile > 0) {   // recompression: flops from the rank logs
+      std::vector<int> r0(c->ntile), rf(c->ntile), lg(c->nlog);
+      for (int s = 0; s < m; ++s) {
+        const SlotPtrs& p = c->slots[s].p;
+        int stv[ST_COUNT];
+        HIPCHK(hipMemcpy(stv, p.status, sizeof(stv), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(r0.data(), p.rank0, c->ntile * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(rf.data(), p.rank, c->ntile * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(lg.data(), p.rlog, c->nlog * sizeof(int), hipMemcpyDeviceToHost));
+        const int fa = stv[ST_FAILED] == 0x7fffffff ? -1 : stv[ST_FAILED];
+        c->timed_flops += recompress_flops(c, r0, rf, lg, fa);
+      }
+    }
   }
   int ret = TLR_OK;
   std::vector<int> rk;
@@ -514,7 +568,7 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
 }
 
 int tlr_ctx_set_timing(tlr_ctx* ctx, int32_t kernel_class) {
-  if (!ctx || kernel_class < 0 || kernel_class > 4) return set_err(TLR_ERR_ARG, "invalid kernel class");
+  if (!ctx || kernel_class < 0 || kernel_class > 5) return set_err(TLR_ERR_ARG, "invalid kernel class");
   ctx->timing_kernel = kernel_class;
   ctx->timed_ms = 0.0;
   ctx->timed_flops = 0.0;
@@ -530,6 +584,18 @@ int tlr_ctx_get_timing(const tlr_ctx* ctx, double* total_ms, int64_t* launches, 
   return TLR_OK;
 }
 
+int tlr_ctx_precisions(const tlr_ctx* ctx, int32_t slot, int32_t* prec) {
+  if (!ctx || !prec || slot < 0 || slot >= ctx->max_batch) return set_err(TLR_ERR_ARG, "invalid argument");
+  const int nB = ctx->nB;
+  memset(prec, 0, sizeof(int32_t) * nB * nB);
+  if (ctx->ntile == 0) return TLR_OK;
+  std::vector<int> pv(ctx->ntile);
+  HIPCHK(hipMemcpy(pv.data(), ctx->slots[slot].p.prec, ctx->ntile * sizeof(int), hipMemcpyDeviceToHost));
+  for (int i = 1; i < nB; ++i)
+    for (int j = 0; j < i; ++j) prec[i * nB + j] = pv[tri_index(i, j)];
+  return TLR_OK;
+}
+
 extern "C" {
 
 int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
@@ -541,6 +607,7 @@ int tlr_loglik_grad_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
                           double h, double* out, double* grad, int32_t* status, int32_t* failed_at,
                           int32_t* ranks_sigma, int32_t* ranks_factor) {
   if (!c || !grad || !(h > 0.0) || h > 0.1) return set_err(TLR_ERR_ARG, "invalid argument");
+  if (rule == TLR_RULE_MIXED) return set_err(TLR_ERR_ARG, "the gradient is not available for the mixed rule");
   std::vector<int32_t> st(m > 0 ? m : 1);
   int rc = run_batch(c, m, thetas, rule, value, precision, nullptr, out, st.data(), failed_at, ranks_sigma,
                      ranks_factor);

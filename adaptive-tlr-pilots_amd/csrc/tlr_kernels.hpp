@@ -33,7 +33,8 @@ struct SlotPtrs {
   double* logdet_part;
   int* status;
   double* result;   // [loglik, logdet, quad]
-  int* rlog;          // rank chosen by every recompression, at rlog_off[k] + pair index
+  int* rlog;
+  int* prec;          // storage precision (8, 16, 32, 64) of every factor tile, mixed rule          // rank chosen by every recompression, at rlog_off[k] + pair index
 };
 
 struct SlotArgs {
@@ -46,6 +47,7 @@ struct SlotArgs {
   int* wl_count;        // nB x RCLASSES entry counts (zeroed at the start of every evaluation)
   int64_t wl_cap;
   int rule_fixed;       // 1: fixed relative tolerance, 0: adaptive absolute budget
+  int mixed;            // 1: adaptive budget plus per-pair storage precision (rounded factors)
   const double* thr;    // per slot: (thr_rel, thr_abs)
   const int* replay;    // per slot: -1, or the record whose ranks this evaluation reuses (frozen ranks)
   const int* rep_rlog;  // records: recompression ranks, nlog per record
@@ -67,11 +69,11 @@ hipError_t launch_residual(const double* sx, const double* sy, int64_t n, int b,
                            const double* A, const double* B, int64_t stride, int k, double* part, const int* active,
                            hipStream_t s);
 size_t rank_decide_lds(int k);
-hipError_t launch_rank_decide(const double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
+hipError_t launch_rank_decide(double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
                               int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
                               double thr_rel, double thr_abs, double* U, double* V, int64_t fstride, int rank_cap,
-                              int* rank, int* need, int* retry_count, int* status, const int* forced, int64_t ntile,
-                              hipStream_t s);
+                              int* rank, int* need, int* retry_count, int* status, const int* forced, int mixed,
+                              int64_t ntile, hipStream_t s);
 hipError_t launch_fill_int(int* p, int64_t count, int value, hipStream_t s);
 hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t s);
 

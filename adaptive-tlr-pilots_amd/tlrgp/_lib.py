@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(HERE, "libtlrgp.so")
 
 TLR_OK, TLR_NOT_PD = 0, 1
 TLR_ERR_ARG, TLR_ERR_HIP, TLR_ERR_RANKCAP, TLR_ERR_NODEVICE = -1, -2, -3, -4
-TLR_RULE_FIXED, TLR_RULE_ADAPTIVE = 0, 1
+TLR_RULE_FIXED, TLR_RULE_ADAPTIVE, TLR_RULE_MIXED = 0, 1, 2
 TLR_PREC_FP64 = 0
 
 # every entry point of include/tlrgp.h with its ctypes signature
@@ -35,6 +35,7 @@ SIGNATURES = {
                                    _D, _D, _D]),
     "tlr_ctx_set_timing": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tlr_ctx_get_timing": (ctypes.c_int, [_VP, _D, _I64, _D]),
+    "tlr_ctx_precisions": (ctypes.c_int, [_VP, ctypes.c_int32, _I]),
     "tlr_last_error": (ctypes.c_char_p, []),
     "tlr_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32]),
 }

@@ -21,7 +21,8 @@ import numpy as np
 from . import _lib
 from ._lib import check, dptr, f64, iptr, lib
 
-RULES = {"fixed": _lib.TLR_RULE_FIXED, "adaptive": _lib.TLR_RULE_ADAPTIVE}
+RULES = {"fixed": _lib.TLR_RULE_FIXED, "adaptive": _lib.TLR_RULE_ADAPTIVE, "mixed": _lib.TLR_RULE_MIXED}
+PRECISION_BITS = (8, 16, 32, 64)
 PRECISIONS = {"fp64": _lib.TLR_PREC_FP64}
 
 
@@ -92,6 +93,20 @@ def rank_info(ranks_sigma, ranks_factor, n, b, completed):
     return info
 
 
+def mixed_info(prec, ranks_factor, n, b):
+    """The reference's byte accounting of the mixed rule (17:210-214): diagonal tiles dense FP64, each factor
+    pair 2 r rows_i bits / 8 bytes in its storage precision."""
+    bounds = tile_bounds(n, b)
+    nb = len(bounds)
+    sizes = np.array([e - s for s, e in bounds])
+    il = np.tril_indices(nb, -1)
+    pr = prec[il]
+    byt = int((sizes ** 2).sum() * 8 + (2 * ranks_factor[il] * sizes[il[0]] * pr // 8).sum())
+    dense = n * (n + 1) // 2 * 8
+    return dict(stored_factor_bytes=byt, dense_factor_bytes=int(dense), factor_bytes_frac=byt / dense,
+                precision_mix_factor_tiles={p: int((pr == p).sum()) for p in PRECISION_BITS}, precision_factor=prec)
+
+
 class TLRContext:
     """Device-resident sites and observations plus workspace for up to max_batch simultaneous theta."""
 
@@ -121,8 +136,8 @@ class TLRContext:
             pass
 
     def set_timing(self, kernel_class):
-        """Bracket launches of one kernel class (1 products, 2 residual, 3 panel QR, 4 rank decision) with
-        HIP events; resets the accumulators."""
+        """Bracket launches of one kernel class with HIP events (1 Matern products, 2 residual, 3 panel QR,
+        5 recompressions of each factorization step, 0 off); resets the accumulators."""
         check(lib.tlr_ctx_set_timing(self._ptr, int(kernel_class)))
 
     def get_timing(self):
@@ -150,8 +165,18 @@ class TLRContext:
         rf = np.zeros((m, self.nB, self.nB), np.int32) if ranks else None
         check(lib.tlr_loglik_batch(self._ptr, m, dptr(th), RULES[rule], float(value), PRECISIONS[precision],
                                    dptr(out), iptr(st), iptr(fa), iptr(rs), iptr(rf)))
-        return dict(loglik=out[:, 0], logdet=out[:, 1], quad=out[:, 2], completed=st == _lib.TLR_OK,
-                    failed_at=fa, ranks_sigma=rs, ranks_factor=rf)
+        res = dict(loglik=out[:, 0], logdet=out[:, 1], quad=out[:, 2], completed=st == _lib.TLR_OK,
+                   failed_at=fa, ranks_sigma=rs, ranks_factor=rf)
+        if rule == "mixed":
+            res["precisions"] = np.stack([self.precisions(s) for s in range(m)])
+        return res
+
+    def precisions(self, slot=0):
+        """Storage precision in bits of every factor tile (nB x nB, 0 off the lower triangle) of the last
+        evaluation in batch slot `slot` with the mixed rule (PREC_OF of 17:38)."""
+        pr = np.zeros((self.nB, self.nB), np.int32)
+        check(lib.tlr_ctx_precisions(self._ptr, int(slot), iptr(pr)))
+        return pr
 
     def loglik_grad_batch(self, thetas, rule, value, h=1e-5, precision="fp64", ranks=True):
         """As loglik_batch plus grad (m, 3): d log L / d(log sigma2, log range, log eta) at frozen ranks (the
@@ -182,6 +207,8 @@ class TLRContext:
             info["failed_at"] = int(r["failed_at"][0])
             return None, None, info
         info.update(logdet=float(r["logdet"][0]), quad=float(r["quad"][0]))
+        if rule == "mixed":
+            info.update(mixed_info(r["precisions"][0], r["ranks_factor"][0], self.n, self.b))
         fac = TLRFactor(r["ranks_factor"][0], info["logdet"], info["quad"], self, np.asarray(theta, float).copy(),
                         rule, value, precision)
         return float(r["loglik"][0]), fac, info

@@ -10,9 +10,10 @@ theta (log L, log-det, quadratic form, per-tile ranks; gradient at frozen ranks:
 closed form, range and eta by central differences that replay the recorded ranks, i.e. four further
 evaluations per theta); value = log L + gradient evaluations per second.
 
-Also reported: the roofline of the dominant kernel (the Matern products of the compression, timed with
-HIP events on the library's stream inside the timed region) and a CPU baseline (the NumPy/SciPy oracle
-timed on the host cores on a bounded sample of the same workload).
+Also reported: the roofline of the dominant kernel (the recompressions of each factorization step, timed
+with HIP events around each step's launches inside the timed region, algorithmic flops from the rank
+logs), the roofline of the compression's Matern products (one extra pass after the timed steps), and a
+CPU baseline (the NumPy/SciPy oracle timed on the host cores on a bounded sample of the same workload).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -28,6 +29,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "adaptive-tlr-pilots_amd"))
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix), AMD published
+# HBM bytes per recompression step (FETCH_SIZE + WRITE_SIZE of the step's dispatches, averaged over the
+# 78 steps of one batch of 20 theta and divided by 20, calibrated against a device copy of known size);
+# measured with rocprofv3 --pmc on this workload, see profiles/recompress_n40000_b500_pmc_traffic.md
+RECOMPRESS_TRAFFIC_PER_STEP = None
 
 
 def lhs_thetas(m, seed=0):
@@ -118,19 +123,25 @@ def main():
     for w in range(a.warmup):
         r = step()
         print(f"# warmup {w}: completed {int(r['completed'].sum())}/{a.batch}", file=sys.stderr, flush=True)
-    ctx.set_timing(1)   # Matern products of the compression: the dominant compression kernel
+    ctx.set_timing(5)   # recompressions of every factorization step: the dominant kernel
     t0 = time.perf_counter()
     for s in range(a.steps):
         r = step()
         print(f"# step {s}: {time.perf_counter() - t0:.3f} s", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     kms, launches, flops = ctx.get_timing()
+    # one more (untimed for throughput) pass timing the Matern products of the compression
+    ctx.set_timing(1)
+    step()
+    pms, plaunches, pflops = ctx.get_timing()
+    ctx.set_timing(0)
     ms_step = 1e3 * elapsed / a.steps
     evals = a.batch * a.steps / elapsed
     nb = (a.n + a.b - 1) // a.b
     il = np.tril_indices(nb, -1)
     mean_rank = float(r["ranks_factor"][:, il[0], il[1]].mean())
     achieved = flops / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
+    p_achieved = pflops / (pms * 1e-3) / 1e12 if pms > 0 else 0.0
     out = {
         "metric": "TLR logL+grad evals/s at N=40k, mixed vs plain FP64 GPU, |dlogL|<=1e-2",
         "value": evals, "unit": "evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -143,8 +154,13 @@ def main():
                    if grad else None, "mixed_precision": None,
                    "completed": int(r["completed"].sum()), "mean_rank_factor": mean_rank},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_matern_product", "launches": launches, "kernel_ms": kms},
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": RECOMPRESS_TRAFFIC_PER_STEP,
+                     "kernel": "k_recompress (work lists + 4 width classes, one launch group per step)",
+                     "launches": launches, "kernel_ms": kms,
+                     "flops_per_launch": flops / max(launches, 1)},
+        "roofline_compression": {"bound": "mfma", "achieved": p_achieved, "peak": FP64_PEAK_TFLOPS,
+                                 "unit": "TFLOP/s", "frac": p_achieved / FP64_PEAK_TFLOPS,
+                                 "kernel": "k_matern_product", "launches": plaunches, "kernel_ms": pms},
     }
     if a.cpu_sample:
         out["cpu_baseline"] = cpu_baseline(S, y, a.b, tuple(thetas[0]), a.rule, a.value, a.cpu_sample,

@@ -16,7 +16,10 @@
  *    one possibly short; the rank rule uses the maximum tile size b (17:143).
  *  - theta = (sigma2, range, eta); the covariance is sigma2 * matern32(h, range) + sigma2 * eta * I.
  *  - rule TLR_RULE_FIXED: value = eps (relative Frobenius tolerance per tile, 17:114, 17:131);
- *    rule TLR_RULE_ADAPTIVE: value = tau, absolute budget tau * b * sigma2 * eta / N (17:150).
+ *    rule TLR_RULE_ADAPTIVE: value = tau, absolute budget tau * b * sigma2 * eta / N (17:150);
+ *    rule TLR_RULE_MIXED: the adaptive budget, and every factor pair stored in the lowest of FP8-E4M3,
+ *    FP16, FP32, FP64 whose rounding error fits the budget left after truncation (17:37-82, 146-147,
+ *    163-164, 183-184, 203-204); arithmetic stays FP64.
  *  - Rank arrays are dense nB x nB int32, row-major, entry [i * nB + j] for the tile (i, j), i > j;
  *    other entries are 0.  ranks_sigma: after the initial compression; ranks_factor: of the factor L.
  *  - Return values: TLR_OK when the factorization completed; TLR_NOT_PD when a diagonal tile was not
@@ -42,6 +45,7 @@ extern "C" {
 
 #define TLR_RULE_FIXED 0
 #define TLR_RULE_ADAPTIVE 1
+#define TLR_RULE_MIXED 2
 
 #define TLR_PREC_FP64 0
 
@@ -95,11 +99,18 @@ int tlr_predict(tlr_ctx* ctx, const double theta[3], int32_t rule, double value,
 
 /* Timing of one kernel class with HIP events on the context's stream, for roofline reports (no
  * counterpart in the reference, which times whole evaluations with time.time(), 17:359-362).
- * kernel_class: 0 off, 1 Matern products, 2 residual, 3 panel QR, 4 rank decision.  Setting it resets the
- * accumulators; tlr_ctx_get_timing returns the summed kernel time, the number of launches and the
- * algorithmic flops of those launches (products: 2 b^2 k per processed tile). */
+ * kernel_class: 0 off, 1 Matern products, 2 residual, 3 panel QR, 5 the recompressions of each
+ * factorization step (work lists and the four width classes, from the first launch to the join).  Setting
+ * it resets the accumulators; tlr_ctx_get_timing returns the summed time, the number of timed launches
+ * (steps for class 5) and their algorithmic flops (products: 2 b^2 k per processed tile; recompression:
+ * 4 b r_ik r_jk + 8 b w^2 + 22 w^3 + 4 b w r' per pair, w = r_ij + r_jk, from the rank logs). */
 int tlr_ctx_set_timing(tlr_ctx* ctx, int32_t kernel_class);
 int tlr_ctx_get_timing(const tlr_ctx* ctx, double* total_ms, int64_t* launches, double* flops);
+
+/* Storage precision (8, 16, 32 or 64 bits) of every factor tile of the last evaluation in batch slot
+ * `slot` under TLR_RULE_MIXED: prec is nB x nB int32 row-major, entry [i * nB + j] for the tile (i, j),
+ * 0 elsewhere.  Replaces PREC_OF and info["precision_mix_factor_tiles"] of 17:38, 210-214. */
+int tlr_ctx_precisions(const tlr_ctx* ctx, int32_t slot, int32_t* prec);
 
 /* Message of the last error on the calling thread. */
 const char* tlr_last_error(void);
