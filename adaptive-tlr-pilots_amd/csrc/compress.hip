@@ -341,8 +341,40 @@ __global__ void k_copy_int(int* dst, const int* src, int64_t count) {
     dst[e] = src[e];
 }
 
+// Compact copy of the initially compressed factors of one slot: tile t's first r0[t] columns of U and V
+// (b rows each) live at stash + 2 b off[t] (U) and stash + 2 b off[t] + b r0[t] (V), off = exclusive prefix
+// sum of r0.  dir 0: slot -> stash; dir 1: stash -> slot, and rank[t] = rank0[t] = r0[t].
+__global__ void k_stash(double* __restrict__ U, double* __restrict__ V, int64_t fstride, int b,
+                        const int* __restrict__ r0, const int64_t* __restrict__ off, double* __restrict__ stash,
+                        int* __restrict__ rank, int* __restrict__ rank0, int dir) {
+  const int64_t t = blockIdx.y;
+  const int r = r0[t];
+  double* st = stash + 2 * (int64_t)b * off[t];
+  double* Ut = U + t * fstride;
+  double* Vt = V + t * fstride;
+  const int64_t cnt = (int64_t)b * r;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += (int64_t)gridDim.x * blockDim.x) {
+    if (dir == 0) {
+      st[e] = Ut[e];
+      st[cnt + e] = Vt[e];
+    } else {
+      Ut[e] = st[e];
+      Vt[e] = st[cnt + e];
+    }
+  }
+  if (dir == 1 && blockIdx.x == 0 && threadIdx.x == 0) { rank[t] = r; rank0[t] = r; }
+}
+
 // ------------------------------------------------------------------------------------ launchers
 static int nblk(int64_t a, int64_t d) { return (int)((a + d - 1) / d); }
+
+hipError_t launch_stash(double* U, double* V, int64_t fstride, int b, const int* r0, const int64_t* off,
+                        double* stash, int* rank, int* rank0, int64_t ntile, int dir, hipStream_t s) {
+  if (ntile <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stash, dim3(4, (unsigned)ntile), dim3(256), 0, s, U, V, fstride, b, r0, off, stash, rank, rank0,
+                     dir);
+  return hipGetLastError();
+}
 
 hipError_t launch_fill_int(int* p, int64_t count, int value, hipStream_t s) {
   if (count <= 0) return hipSuccess;

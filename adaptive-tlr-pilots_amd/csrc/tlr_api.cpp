@@ -86,6 +86,11 @@ struct tlr_ctx {
   int64_t* d_rlog_off = nullptr;       // nB offsets into a rank log
   int* rep_rank0 = nullptr;            // max_batch records of initial ranks (ntile each)
   int* rep_rlog = nullptr;             // max_batch records of recompression ranks (nlog each)
+  // initially compressed factors of the replay records (compact: the first rank0 columns of each tile),
+  // reused by perturbations that leave the off-diagonal tiles unchanged (eta)
+  double* stash = nullptr;
+  int64_t stash_cap = 0;               // doubles
+  int64_t* d_stash_off = nullptr;      // max_batch x ntile column offsets into the stash
   // prediction workspace, grown on demand: N x m cross covariances, alpha, prediction sites, reductions
   double* pred_buf = nullptr;
   int64_t pred_cap = 0;
@@ -98,6 +103,7 @@ struct tlr_ctx {
   int64_t timed_launches = 0;
   ~tlr_ctx() {
     if (pred_buf) (void)hipFree(pred_buf);
+    if (stash) (void)hipFree(stash);
     if (h_thr) (void)hipHostFree(h_thr);
     if (h_replay) (void)hipHostFree(h_replay);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
@@ -298,6 +304,7 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
   for (int s = 0; s < max_batch; ++s) ACHK(c->alloc(&c->slots[s].p.rlog, (size_t)c->nlog));
   ACHK(c->alloc(&c->rep_rank0, (size_t)max_batch * nt));
   ACHK(c->alloc(&c->rep_rlog, (size_t)max_batch * c->nlog));
+  ACHK(c->alloc(&c->d_stash_off, (size_t)max_batch * nt));
   ACHK(c->alloc(&c->d_replay, (size_t)max_batch));
   ACHK(hipHostMalloc((void**)&c->h_replay, sizeof(int) * max_batch, hipHostMallocDefault));
   ACHK(c->alloc(&c->d_slots, (size_t)max_batch));
@@ -445,10 +452,43 @@ static double recompress_flops(const tlr_ctx* c, const std::vector<int>& rank0, 
   return fl;
 }
 
-// One pass over m theta; replay[s] >= 0 makes slot s reuse the ranks of record replay[s] (frozen ranks).
+// Copy the initially compressed factors of slots 0..m-1 into stash records 0..m-1 (compact, offsets from
+// each slot's initial ranks).  Called between the compression and the factorization.
+static int stash_compressed(tlr_ctx* c, int m) {
+  const int64_t nt = c->ntile;
+  if (nt == 0) return TLR_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<int> r0(nt);
+  std::vector<int64_t> off((size_t)m * nt);
+  int64_t cols = 0;
+  for (int s = 0; s < m; ++s) {
+    HIPCHK(hipMemcpy(r0.data(), c->slots[s].p.rank0, nt * sizeof(int), hipMemcpyDeviceToHost));
+    for (int64_t t = 0; t < nt; ++t) { off[(size_t)s * nt + t] = cols; cols += r0[t]; }
+  }
+  const int64_t need = 2 * (int64_t)c->b * (cols > 0 ? cols : 1);
+  if (need > c->stash_cap) {
+    if (c->stash) HIPCHK(hipFree(c->stash));
+    c->stash = nullptr;
+    c->stash_cap = 0;
+    HIPCHK(hipMalloc(&c->stash, need * sizeof(double)));
+    c->stash_cap = need;
+  }
+  HIPCHK(hipMemcpy(c->d_stash_off, off.data(), (size_t)m * nt * sizeof(int64_t), hipMemcpyHostToDevice));
+  for (int s = 0; s < m; ++s) {
+    const SlotPtrs& p = c->slots[s].p;
+    HIPCHK(launch_stash(p.U, p.V, (int64_t)c->b * RANK_CAP, c->b, p.rank0, c->d_stash_off + (int64_t)s * nt, c->stash,
+                        p.rank, p.rank0, nt, 0, c->stream));
+  }
+  return TLR_OK;
+}
+
+// One pass over m theta; replay[s] >= 0 makes slot s reuse the ranks of record replay[s] (frozen ranks);
+// reuse[s] >= 0 makes it also take record reuse[s]'s stashed compressed factors instead of compressing
+// (valid when theta differs from the record's only in eta, which leaves the off-diagonal tiles unchanged);
+// stash != 0 stores every slot's compressed factors as records 0..m-1.
 static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
                      const int* replay, double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma,
-                     int32_t* ranks_factor) {
+                     int32_t* ranks_factor, const int* reuse = nullptr, int stash = 0) {
   if (!c || !thetas || !out || m < 1 || m > c->max_batch) return set_err(TLR_ERR_ARG, "invalid argument");
   if (rule != TLR_RULE_FIXED && rule != TLR_RULE_ADAPTIVE && rule != TLR_RULE_MIXED)
     return set_err(TLR_ERR_ARG, "unknown rule");
@@ -491,7 +531,18 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
     HIPCHK(hipMemcpyAsync(p.rhs, c->y, c->n * sizeof(double), hipMemcpyDeviceToDevice, st));
     const int* forced = (replay && replay[s] >= 0) ? c->rep_rank0 + (int64_t)replay[s] * c->ntile : nullptr;
     if (mixed) HIPCHK(launch_fill_int(p.prec, c->ntile, 64, st));
-    int rc = compress_slot(c, s, p, mp, rule_fixed, value, thr_abs, forced, mixed);
+    if (reuse && reuse[s] >= 0) {
+      const int rec = reuse[s];
+      HIPCHK(launch_gen_diag(c->sx, c->sy, c->n, c->b, c->nB, mp, p.D, st));
+      HIPCHK(launch_stash(p.U, p.V, (int64_t)c->b * RANK_CAP, c->b, c->rep_rank0 + (int64_t)rec * c->ntile,
+                          c->d_stash_off + (int64_t)rec * c->ntile, c->stash, p.rank, p.rank0, c->ntile, 1, st));
+    } else {
+      int rc = compress_slot(c, s, p, mp, rule_fixed, value, thr_abs, forced, mixed);
+      if (rc != TLR_OK) return rc;
+    }
+  }
+  if (stash) {
+    const int rc = stash_compressed(c, m);
     if (rc != TLR_OK) return rc;
   }
   SlotArgs sa = c->args();
@@ -610,7 +661,7 @@ int tlr_loglik_grad_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
   if (rule == TLR_RULE_MIXED) return set_err(TLR_ERR_ARG, "the gradient is not available for the mixed rule");
   std::vector<int32_t> st(m > 0 ? m : 1);
   int rc = run_batch(c, m, thetas, rule, value, precision, nullptr, out, st.data(), failed_at, ranks_sigma,
-                     ranks_factor);
+                     ranks_factor, nullptr, 1);
   if (rc < 0) return rc;
   if (status) memcpy(status, st.data(), sizeof(int32_t) * m);
   // keep the base ranks as replay records
@@ -625,9 +676,10 @@ int tlr_loglik_grad_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
   HIPCHK(hipStreamSynchronize(c->stream));
   // d/dlog sigma2 in closed form: tiles and budgets both scale with sigma2, so the ranks are unchanged, L
   // scales by sigma and log L = -(quad / sigma2 + logdet + N log sigma2 + N log 2 pi) / 2 in sigma2
-  // d/dlog range, d/dlog eta: central differences at the base ranks
+  // d/dlog range, d/dlog eta: central differences at the base ranks; the eta perturbations leave the
+  // off-diagonal tiles unchanged, so they reuse the base's compressed factors
   std::vector<double> th4;
-  std::vector<int> src;
+  std::vector<int> src, reuse4;
   for (int s = 0; s < m; ++s) {
     grad[3 * s + 0] = st[s] == TLR_OK ? 0.5 * out[3 * s + 2] - 0.5 * (double)c->n : NAN;
     grad[3 * s + 1] = grad[3 * s + 2] = NAN;
@@ -638,6 +690,7 @@ int tlr_loglik_grad_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
         th4.insert(th4.end(), {t[0], t[1], t[2]});
         th4[th4.size() - 3 + par] *= exp(sg * h);
         src.push_back(s);
+        reuse4.push_back(par == 2 ? s : -1);
       }
   }
   const int n4 = (int)src.size();
@@ -646,7 +699,7 @@ int tlr_loglik_grad_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
   for (int off = 0; off < n4; off += c->max_batch) {
     const int mm = n4 - off < c->max_batch ? n4 - off : c->max_batch;
     rc = run_batch(c, mm, th4.data() + 3 * off, rule, value, precision, src.data() + off, o4.data() + 3 * off,
-                   s4.data() + off, f4.data() + off, nullptr, nullptr);
+                   s4.data() + off, f4.data() + off, nullptr, nullptr, reuse4.data() + off, 0);
     if (rc < 0) return rc;
   }
   for (int e = 0; e < n4; e += 4) {

@@ -75,6 +75,8 @@ hipError_t launch_rank_decide(double* Rm, int64_t rstride, const double* Qrow, c
                               int* rank, int* need, int* retry_count, int* status, const int* forced, int mixed,
                               int64_t ntile, hipStream_t s);
 hipError_t launch_fill_int(int* p, int64_t count, int value, hipStream_t s);
+hipError_t launch_stash(double* U, double* V, int64_t fstride, int b, const int* r0, const int64_t* off,
+                        double* stash, int* rank, int* rank0, int64_t ntile, int dir, hipStream_t s);
 hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t s);
 
 // factor.hip
