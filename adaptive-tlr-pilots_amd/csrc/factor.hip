@@ -392,18 +392,21 @@ __global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
 // (block Gram-Schmidt applied twice, Householder QR of the r_jk new columns only).  The core
 // R_u R_v^T (w x w, w = r_ij + r_jk) goes to the one-sided Jacobi SVD; the outputs are formed row block
 // by row block through LDS, in place.  The panel holds the r_jk new columns in NW waves x CPW column
-// slots: r_jk <= 8 (nine pairs in ten) in 4 waves x 2 slots with a 43 KB LDS footprint, so that three
-// work groups share a CU; r_jk <= 16 in 4 x 4 (two per CU); r_jk <= 32 in 4 x 8; r_jk <= 64 in 8 x 8
-// (their cores up to 64 x 64 in LDS).  The pairs of each class come from a work list.
+// slots: r_jk <= 8 (nine pairs in ten) in 4 waves x 2 slots with a 52 KB LDS footprint, so that three
+// work groups share a CU; r_jk <= 16 in 4 x 4 (two per CU); r_jk <= 32 in 8 x 4 and r_jk <= 64 in 8 x 8
+// (one per CU, cores up to 64 x 64 in LDS).  The pairs of each class come from a work list.
 template <int NW, int CPW>
 struct RecompLayout {
   static constexpr int R2MAX = CPW * NW;
   static constexpr int WMAX = 64 + R2MAX;
   // cores up to WJ x WJ run in LDS (larger ones in the work group's scratch)
-  static constexpr int WJ = R2MAX <= 16 ? 32 : 64;
-  static constexpr bool RR_LDS = WMAX * R2MAX <= 3072;
+  // The 8-wave classes hold a CU's whole register file, so nothing else shares their CU: they take the
+  // LDS as well (cores up to 64 x 64); the 4-wave classes keep to 52-63 KB (cores up to 32 x 32).
+  static constexpr int WJ = NW >= 8 ? 64 : 32;
+  static constexpr bool RR_LDS = R2MAX <= 32;
   // front: W and the CGS coefficients (2 x 64 x R2MAX), later the output staging (RB rows x w)
-  static constexpr int FRONT = 2 * 64 * R2MAX > 2048 ? 2 * 64 * R2MAX : 2048;
+  static constexpr bool WC_LDS = true;
+  static constexpr int FRONT = (WC_LDS && 2 * 64 * R2MAX > 2048) ? 2 * 64 * R2MAX : 2048;
   static constexpr int RR = RR_LDS ? WMAX * R2MAX : 0;
   static constexpr int OFF_RV = FRONT, OFF_RU = OFF_RV + RR, OFF_SC = OFF_RU + RR, OFF_SG = OFF_SC + 64,
                        OFF_G = OFF_SG + 64, OFF_J = OFF_G + WJ * WJ, TOTAL = OFF_J + WJ * WJ;
@@ -449,8 +452,9 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
   const int w = r1 + r2;
   const double thr_rel = sa.thr[2 * slot], thr_abs = sa.thr[2 * slot + 1];
   extern __shared__ double lds[];
-  double* Wm = lds;                           // rik x r2          (phases 1-3)
-  double* Ct = lds + 64 * L::R2MAX;           // r1 x r2 temporary (phases 2-3)
+  // rik x r2 (phases 1-3) and the r1 x r2 temporary (phases 2-3): LDS, or scratch past the other regions
+  double* Wm = L::WC_LDS ? lds : scr + (int64_t)b * 64 + 2 * 128 * 64 + 2 * 128 * 128;
+  double* Ct = L::WC_LDS ? lds + 64 * L::R2MAX : Wm + 64 * 64;
   double* Sb = lds;                           // RB x w staging     (phase 5, aliases the front)
   double* Q2v = scr;                          // b x r2
   double* RvR = L::RR_LDS ? lds + L::OFF_RV : scr + (int64_t)b * 64;            // w x r2: [Cv; R2v]
@@ -706,12 +710,12 @@ hipError_t recompress_occupancy(int b, int blocks_per_cu[RCLASSES]) {
   if (b <= 256) {
     if ((e = occ_t<4, 4, 2>(blocks_per_cu + 0)) != hipSuccess) return e;
     if ((e = occ_t<4, 4, 4>(blocks_per_cu + 1)) != hipSuccess) return e;
-    if ((e = occ_t<4, 4, 8>(blocks_per_cu + 2)) != hipSuccess) return e;
+    if ((e = occ_t<4, 8, 4>(blocks_per_cu + 2)) != hipSuccess) return e;
     return occ_t<4, 8, 8>(blocks_per_cu + 3);
   }
   if ((e = occ_t<8, 4, 2>(blocks_per_cu + 0)) != hipSuccess) return e;
   if ((e = occ_t<8, 4, 4>(blocks_per_cu + 1)) != hipSuccess) return e;
-  if ((e = occ_t<8, 4, 8>(blocks_per_cu + 2)) != hipSuccess) return e;
+  if ((e = occ_t<8, 8, 4>(blocks_per_cu + 2)) != hipSuccess) return e;
   return occ_t<8, 8, 8>(blocks_per_cu + 3);
 }
 
@@ -733,14 +737,14 @@ hipError_t launch_recomp_class(const SlotArgs& sa_in, int k, int cls, int grid, 
     switch (cls) {
       case 0: return launch_recomp_t<4, 4, 2>(sa, k, 0, grid, s);
       case 1: return launch_recomp_t<4, 4, 4>(sa, k, 1, grid, s);
-      case 2: return launch_recomp_t<4, 4, 8>(sa, k, 2, grid, s);
+      case 2: return launch_recomp_t<4, 8, 4>(sa, k, 2, grid, s);
       default: return launch_recomp_t<4, 8, 8>(sa, k, 3, grid, s);
     }
   }
   switch (cls) {
     case 0: return launch_recomp_t<8, 4, 2>(sa, k, 0, grid, s);
     case 1: return launch_recomp_t<8, 4, 4>(sa, k, 1, grid, s);
-    case 2: return launch_recomp_t<8, 4, 8>(sa, k, 2, grid, s);
+    case 2: return launch_recomp_t<8, 8, 4>(sa, k, 2, grid, s);
     default: return launch_recomp_t<8, 8, 8>(sa, k, 3, grid, s);
   }
 }

@@ -241,8 +241,9 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
   ACHK(c->alloc(&c->need, (size_t)nt));
   // recompression: a persistent grid per width class sized to the resident capacity of the device; each
   // resident work group owns scratch for Q2v (b x 64), the right blocks of R_u, R_v (128 x 64 each) and
-  // the Jacobi core and rotations (128 x 128 each) when they do not fit in LDS
-  c->scr_stride = (int64_t)b * 64 + 2 * WCAP * 64 + 2 * WCAP * WCAP;
+  // the Jacobi core and rotations (128 x 128 each) when they do not fit in LDS, W and the projection
+  // coefficients of the widest class (64 x 64 each)
+  c->scr_stride = (int64_t)b * 64 + 2 * WCAP * 64 + 2 * WCAP * WCAP + 2 * 64 * 64;
   {
     int dev = 0, cus = 0, per_cu[RCLASSES];
     ACHK(hipGetDevice(&dev));
@@ -261,7 +262,7 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
       off += c->recomp_grid[q];
     }
     c->cls_stream[0] = c->stream;
-    for (int q = 1; q < RCLASSES; ++q) {
+    for (int q = 1; q <= 2; ++q) {
       ACHK(hipStreamCreateWithFlags(&c->cls_stream[q], hipStreamNonBlocking));
       ACHK(hipEventCreateWithFlags(&c->ev_cls[q], hipEventDisableTiming));
     }
@@ -400,9 +401,10 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
 
 }  // extern "C"
 
-// Recompressions of step k: the work lists on the main stream, then the four width classes at once, the
-// wide ones on their own streams (their few large pairs take long; the narrow class fills the device
-// meanwhile), joined back into the main stream.
+// Recompressions of step k: the work lists on the main stream, then the width classes at once on three
+// streams (the context has four hardware queues; the null stream takes one): the narrow class on the
+// main stream, 9-16 followed by 33-64 on a second, 17-32 on a third, joined back into the main stream.
+// The few wide pairs of a step take long; the narrow class fills the device meanwhile.
 static hipError_t launch_recompress_step(tlr_ctx* c, const SlotArgs& sa, int k, int nslot) {
   if (c->nB - k - 1 < 2) return hipSuccess;
   TimedLaunch tt{nullptr, nullptr, -1, k, 0.0};
@@ -410,14 +412,19 @@ static hipError_t launch_recompress_step(tlr_ctx* c, const SlotArgs& sa, int k, 
   hipError_t e = launch_recomp_list(sa, k, nslot, c->stream);
   if (e != hipSuccess) return e;
   if ((e = hipEventRecord(c->ev_list, c->stream)) != hipSuccess) return e;
-  for (int q = 1; q < RCLASSES; ++q) {
+  static const int plan[2][2] = {{2, -1}, {1, 3}};   // classes run by side streams 1 and 2, in order
+  for (int q = 1; q <= 2; ++q) {
     if ((e = hipStreamWaitEvent(c->cls_stream[q], c->ev_list, 0)) != hipSuccess) return e;
-    if ((e = launch_recomp_class(sa, k, q, c->recomp_grid[q], c->scr_cls[q], c->cls_stream[q])) != hipSuccess)
-      return e;
+    for (int u = 0; u < 2; ++u) {
+      const int cls = plan[q - 1][u];
+      if (cls < 0) continue;
+      if ((e = launch_recomp_class(sa, k, cls, c->recomp_grid[cls], c->scr_cls[cls], c->cls_stream[q])) != hipSuccess)
+        return e;
+    }
     if ((e = hipEventRecord(c->ev_cls[q], c->cls_stream[q])) != hipSuccess) return e;
   }
   if ((e = launch_recomp_class(sa, k, 0, c->recomp_grid[0], c->scr_cls[0], c->stream)) != hipSuccess) return e;
-  for (int q = 1; q < RCLASSES; ++q)
+  for (int q = 1; q <= 2; ++q)
     if ((e = hipStreamWaitEvent(c->stream, c->ev_cls[q], 0)) != hipSuccess) return e;
   c->toc(on, tt);
   return hipSuccess;
@@ -771,6 +778,38 @@ int tlr_predict(tlr_ctx* c, const double theta[3], int32_t rule, double value, i
   for (int j = 0; j < m; ++j) {
     const double v = v0 - ss[j];
     sd[j] = sqrt(v > 1e-12 ? v : 1e-12);
+  }
+  return TLR_OK;
+}
+
+int tlr_solve(tlr_ctx* c, const double theta[3], int32_t rule, double value, int32_t precision, double* B,
+              int64_t ldb, int64_t m, int32_t mode, double out3[3]) {
+  if (!c || !theta || !B || m < 1 || ldb < c->n || mode < 0 || mode > 2) return set_err(TLR_ERR_ARG, "invalid argument");
+  double o3[3];
+  int32_t st = 0, fa = -1;
+  int rc = run_batch(c, 1, theta, rule, value, precision, nullptr, o3, &st, &fa, nullptr, nullptr);
+  if (rc < 0) return rc;
+  if (out3) memcpy(out3, o3, sizeof(o3));
+  if (st != TLR_OK) return TLR_NOT_PD;
+  const int64_t n = c->n;
+  const int64_t chunk = m < 4096 ? m : 4096;           // right-hand sides per pass through the device
+  if (n * chunk > c->pred_cap) {
+    if (c->pred_buf) HIPCHK(hipFree(c->pred_buf));
+    c->pred_buf = nullptr;
+    HIPCHK(hipMalloc(&c->pred_buf, n * chunk * sizeof(double)));
+    c->pred_cap = n * chunk;
+  }
+  SlotArgs sa = c->args();
+  hipStream_t s = c->stream;
+  for (int64_t c0 = 0; c0 < m; c0 += chunk) {
+    const int mc = (int)(m - c0 < chunk ? m - c0 : chunk);
+    HIPCHK(hipMemcpy2DAsync(c->pred_buf, n * sizeof(double), B + c0 * ldb, ldb * sizeof(double), n * sizeof(double),
+                            mc, hipMemcpyHostToDevice, s));
+    if (mode != 1) HIPCHK(launch_solve(sa, 0, c->pred_buf, n, mc, 0, s));
+    if (mode != 0) HIPCHK(launch_solve(sa, 0, c->pred_buf, n, mc, 1, s));
+    HIPCHK(hipMemcpy2DAsync(B + c0 * ldb, ldb * sizeof(double), c->pred_buf, n * sizeof(double), n * sizeof(double),
+                            mc, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
   }
   return TLR_OK;
 }

@@ -2,9 +2,11 @@
 
 Importing the package loads libtlrgp.so; there is no CPU fallback."""
 from ._lib import TLRError, device_info, lib  # noqa: F401  (fails loudly when the library is missing)
-from .likelihood import (Lik, TLRContext, TLRFactor, crps_gauss, morton_order, neg_loglik_tlr,  # noqa: F401
-                         rank_info, skill, tile_bounds, tlr_loglik, tlr_loglik_grad, tlr_predict)
+from .likelihood import (Lik, TLRContext, TLRFactor, crps_gauss, kl_tlr, morton_order, neg_loglik_tlr,  # noqa: F401
+                         rank_info, skill, tile_bounds, tlr_backward, tlr_forward, tlr_loglik, tlr_loglik_grad,
+                         tlr_predict)
 from . import inference  # noqa: F401,E402  (MLE, MH, HMC and chain diagnostics over the likelihood)
 
-__all__ = ["Lik", "TLRContext", "TLRFactor", "TLRError", "crps_gauss", "device_info", "morton_order",
-           "neg_loglik_tlr", "rank_info", "skill", "tile_bounds", "tlr_loglik", "tlr_loglik_grad", "tlr_predict"]
+__all__ = ["Lik", "TLRContext", "TLRFactor", "TLRError", "crps_gauss", "device_info", "kl_tlr", "morton_order",
+           "neg_loglik_tlr", "rank_info", "skill", "tile_bounds", "tlr_backward", "tlr_forward", "tlr_loglik",
+           "tlr_loglik_grad", "tlr_predict"]

@@ -227,6 +227,22 @@ class TLRContext:
             return None
         return mu, sd
 
+    def solve(self, theta, rule, value, B, mode="forward", precision="fp64"):
+        """Factorize at theta and return L^{-1} B ("forward", tlr_forward 17:230-240), L^{-T} B ("backward",
+        tlr_backward 17:243-256) or (L L^T)^{-1} B ("both"); B is (N,) or (N, m).  None when the factorization
+        fails.  Second value: (loglik, logdet, quad) of the factorization."""
+        B = np.asarray(B, dtype=np.float64)
+        vec = B.ndim == 1
+        X = np.array(B.reshape(self.n, -1), dtype=np.float64, order="F", copy=True)
+        if X.shape[0] != self.n:
+            raise ValueError(f"B must have {self.n} rows")
+        o3 = np.empty(3)
+        rc = check(lib.tlr_solve(self._ptr, dptr(f64(theta, (3,))), RULES[rule], float(value), PRECISIONS[precision],
+                                 dptr(X), self.n, X.shape[1], {"forward": 0, "backward": 1, "both": 2}[mode], dptr(o3)))
+        if rc == _lib.TLR_NOT_PD:
+            return None, o3
+        return (X[:, 0] if vec else X), o3
+
 
 _CTX_CACHE = {}
 
@@ -295,6 +311,55 @@ def skill(yp, mu, sig):
     z = np.abs(yp - mu) / sig
     return dict(rmse=float(np.sqrt(((yp - mu) ** 2).mean())), crps=float(crps_gauss(yp, mu, sig)),
                 cover90=float((z <= 1.6448536).mean()))
+
+
+def tlr_forward(S, y, theta, bounds, rule, value, B):
+    """L^{-1} B with the TLR factor of Sigma(theta) (17:230-240, which takes the factor itself)."""
+    S = f64(S)
+    ctx = _context_for(S, y, _tile_size_from_bounds(S.shape[0], bounds))
+    return ctx.solve(theta, rule, value, B, "forward")[0]
+
+
+def tlr_backward(S, y, theta, bounds, rule, value, Z):
+    """L^{-T} Z with the TLR factor of Sigma(theta) (17:243-256)."""
+    S = f64(S)
+    ctx = _context_for(S, y, _tile_size_from_bounds(S.shape[0], bounds))
+    return ctx.solve(theta, rule, value, Z, "backward")[0]
+
+
+def kl_tlr(S, theta, bounds, rule, value):
+    """KL(N(0, Sigma) || N(0, Sigma_TLR)) = (tr(Sigma_TLR^{-1} Sigma) - N + logdet Sigma_TLR - logdet Sigma) / 2
+    as codes/31_kl_tlr.py:37-45: the trace is ||L^{-1} C||_F^2 with C the exact dense Cholesky factor of Sigma
+    (computed here on the host with SciPy: it is the exact side of the comparison) and L the GPU TLR factor;
+    the N right-hand sides go through the GPU forward solve in blocks.  Returns the reference's keys."""
+    from scipy.linalg import cholesky
+    S = f64(S)
+    n = S.shape[0]
+    sigma2, r, eta = (float(t) for t in theta)
+    d = np.sqrt(np.maximum(((S[:, None, :] - S[None, :, :]) ** 2).sum(-1), 0.0)) if n <= 4000 else None
+    if d is None:   # row blocks, one N x N array
+        K = np.empty((n, n))
+        for i0 in range(0, n, 2000):
+            i1 = min(n, i0 + 2000)
+            dd = (S[i0:i1, None, 0] - S[None, :, 0]) ** 2 + (S[i0:i1, None, 1] - S[None, :, 1]) ** 2
+            x = np.sqrt(np.maximum(dd, 0.0)) * (np.sqrt(3.0) / r)
+            K[i0:i1] = sigma2 * ((1.0 + x) * np.exp(-x))
+    else:
+        x = d * (np.sqrt(3.0) / r)
+        K = sigma2 * ((1.0 + x) * np.exp(-x))
+    K[np.diag_indices(n)] += sigma2 * eta
+    C = cholesky(K, lower=True, overwrite_a=True, check_finite=False)
+    del K
+    logdet = 2.0 * float(np.log(np.diag(C)).sum())
+    ctx = _context_for(S, np.zeros(n), _tile_size_from_bounds(n, bounds))
+    X, o3 = ctx.solve(theta, rule, value, C, "forward")
+    if X is None:
+        return None
+    del C
+    tr = float((X * X).sum())
+    logdet_t = float(o3[1])
+    return dict(method=f"{rule} TLR", tau=value, kl=0.5 * (tr - n + logdet_t - logdet), trace=tr, logdet_exact=logdet,
+                logdet_approx=logdet_t)
 
 
 def neg_loglik_tlr(logth, S, y, bounds, rule, value, trace):

@@ -97,6 +97,13 @@ int tlr_loglik_grad(tlr_ctx* ctx, const double theta[3], int32_t rule, double va
 int tlr_predict(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, int32_t precision, const double* Sp,
                 int64_t n_pred, double* mu, double* sd, double out3[3]);
 
+/* Factorize at theta and solve with the TLR factor, in place on the host block B (N x m, column-major,
+ * leading dimension ldb >= N): mode 0 B <- L^{-1} B (tlr_forward, 17:230-240), mode 1 B <- L^{-T} B
+ * (tlr_backward, 17:243-256), mode 2 B <- (L L^T)^{-1} B.  out3 receives (loglik, logdet, quad) of the
+ * factorization (may be NULL).  Returns TLR_NOT_PD when the factorization fails (B unchanged). */
+int tlr_solve(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, int32_t precision, double* B,
+              int64_t ldb, int64_t m, int32_t mode, double out3[3]);
+
 /* Timing of one kernel class with HIP events on the context's stream, for roofline reports (no
  * counterpart in the reference, which times whole evaluations with time.time(), 17:359-362).
  * kernel_class: 0 off, 1 Matern products, 2 residual, 3 panel QR, 5 the recompressions of each

@@ -60,8 +60,9 @@ def main(ref="/root/reference"):
         g["tlr_cholesky"][str(n)] = dict(b=d["settings"]["b"], dense_loglik=d["dense_loglik_true"], rules=rules,
                                          dense_skill=d.get("dense_skill_true_theta"))
     kl = _load_json(ref, "screening_stress/nu15_r03_e005/kl_tlr.json")
-    g["kl_tlr_n10000"] = dict(logdet_exact=kl["logdet_exact"], logdet_approx=kl["logdet_approx"],
-                              tau=kl["tau"], note="screening field nu15_r03_e005 = the same N=10k field")
+    g["kl_tlr_n10000"] = dict(logdet_exact=kl["logdet_exact"], logdet_approx=kl["logdet_approx"], kl=kl["kl"],
+                              trace=kl["trace"], tau=kl["tau"],
+                              note="screening field nu15_r03_e005 = the same N=10k field")
     g["bound_terms"] = {}
     for n in (10000, 40000):
         bt = _load_json(ref, f"bound_terms/bound_terms_n{n}.json")
