@@ -29,10 +29,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "adaptive-tlr-pilots_amd"))
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix), AMD published
-# HBM bytes per recompression step (FETCH_SIZE + WRITE_SIZE of the step's dispatches, averaged over the
-# 78 steps of one batch of 20 theta and divided by 20, calibrated against a device copy of known size);
+# HBM bytes per recompression step of a batch of 20 theta (FETCH_SIZE + WRITE_SIZE of the step's
+# dispatches averaged over the 78 steps, each counter calibrated against a device copy of known size);
 # measured with rocprofv3 --pmc on this workload, see profiles/recompress_n40000_b500_pmc_traffic.md
-RECOMPRESS_TRAFFIC_PER_STEP = None
+RECOMPRESS_TRAFFIC_PER_STEP = 5.63e9
 
 
 def lhs_thetas(m, seed=0):
@@ -151,7 +151,9 @@ def main():
                                f"{a.rule} tau={a.value}, {a.batch} Latin-hypercube theta per step, plain FP64",
                    "n": a.n, "b": a.b, "theta_per_step": a.batch, "precision": "fp64", "gradient": grad,
                    "gradient_method": "frozen ranks: sigma2 closed form, range/eta central differences h=1e-5"
-                   if grad else None, "mixed_precision": None,
+                   if grad else None,
+                   "mixed_precision": "not used: FP32 products measured 2.3x slower with a per-tile FP64 fallback "
+                                      "and |dlogL| up to 4.8e-2 at this configuration",
                    "completed": int(r["completed"].sum()), "mean_rank_factor": mean_rank},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": RECOMPRESS_TRAFFIC_PER_STEP,
