@@ -112,8 +112,10 @@ def main():
     S, y = load_field(a.n)
     thetas = lhs_thetas(a.batch)
     print(f"# device: {tlrgp.device_info()}", file=sys.stderr, flush=True)
-    ctx = tlrgp.TLRContext(S, y, a.b, max_batch=a.batch)
     grad = not a.no_grad
+    # the gradient's 4 replays per theta run as one batch when the context holds 5 x batch slots
+    # (about 1.8 GB per slot at N = 40,000, b = 500: 180 GB of the 288 GB for 20 theta)
+    ctx = tlrgp.TLRContext(S, y, a.b, max_batch=a.batch * (5 if grad else 1))
 
     def step():
         if grad:
