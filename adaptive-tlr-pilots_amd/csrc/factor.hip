@@ -416,8 +416,10 @@ struct RecompLayout {
 __device__ __forceinline__ int recomp_class(int r2) { return r2 <= 8 ? 0 : (r2 <= 16 ? 1 : (r2 <= 32 ? 2 : 3)); }
 
 // Work lists of step k: one thread per (pair, slot); pairs with r_ik, r_jk > 0 go to the list of their
-// class.  Ranks outside [0, rank_cap] trip the guard instead.  The order inside a list depends on the
-// atomics; every pair is computed independently of the others, so results do not.
+// class, the heavy ones (wide cores: r_ij + r_jk above the class's threshold) from the front and the rest
+// from the back, so that the persistent grid starts the long pairs first.  Ranks outside [0, rank_cap]
+// trip the guard instead.  The order inside a list depends on the atomics; every pair is computed
+// independently of the others, so results do not.
 __global__ __launch_bounds__(256) void k_recomp_list(SlotArgs sa, int k, int npairs) {
   const int q = blockIdx.x * 256 + threadIdx.x;
   const int slot = blockIdx.y;
@@ -434,8 +436,11 @@ __global__ __launch_bounds__(256) void k_recomp_list(SlotArgs sa, int k, int npa
   }
   if (rik == 0 || r2 == 0) return;
   const int c = recomp_class(r2);
-  const int pos = atomicAdd(&sa.wl_count[RCLASSES * k + c], 1);
-  sa.wl[(int64_t)c * sa.wl_cap + pos] = make_int2(q, slot);
+  const int heavy_w[RCLASSES] = {16, 32, 48, 80};
+  int* cnt = sa.wl_count + 2 * (RCLASSES * k + c);
+  int2* list = sa.wl + (int64_t)c * sa.wl_cap;
+  if (r1 + r2 > heavy_w[c]) list[atomicAdd(cnt, 1)] = make_int2(q, slot);
+  else list[sa.wl_cap - 1 - atomicAdd(cnt + 1, 1)] = make_int2(q, slot);
 }
 
 // One pair (i, j) = pair index q of step k, for slot `slot`; scr is the work group's scratch.
@@ -619,11 +624,12 @@ struct RecompWaves {
 
 template <int RPL, int NW, int CPW>
 __global__ __launch_bounds__(64 * NW, (RecompWaves<NW, CPW>::value)) void k_recompress(SlotArgs sa, int k, int cls) {
-  const int cnt = __builtin_amdgcn_readfirstlane(sa.wl_count[RCLASSES * k + cls]);
+  const int nh = __builtin_amdgcn_readfirstlane(sa.wl_count[2 * (RCLASSES * k + cls)]);
+  const int cnt = nh + __builtin_amdgcn_readfirstlane(sa.wl_count[2 * (RCLASSES * k + cls) + 1]);
   const int2* list = sa.wl + (int64_t)cls * sa.wl_cap;
   double* scr = sa.scr_pool + (int64_t)blockIdx.x * sa.scr_stride;
   for (int q = blockIdx.x; q < cnt; q += gridDim.x) {
-    const int2 e = list[q];
+    const int2 e = list[q < nh ? q : sa.wl_cap - 1 - (q - nh)];
     recompress_pair<RPL, NW, CPW>(sa, k, __builtin_amdgcn_readfirstlane(e.x), __builtin_amdgcn_readfirstlane(e.y),
                                   scr);
     __syncthreads();

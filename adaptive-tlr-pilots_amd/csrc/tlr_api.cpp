@@ -74,7 +74,7 @@ struct tlr_ctx {
   hipStream_t cls_stream[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};   // [0] = stream
   hipEvent_t ev_list = nullptr, ev_cls[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};
   int2* wl = nullptr;                  // recompression work lists (RCLASSES x wl_cap)
-  int* wl_count = nullptr;             // nB x RCLASSES list lengths
+  int* wl_count = nullptr;             // nB x RCLASSES x (heavy, light) list lengths
   int64_t wl_cap = 0;
   int recomp_grid[RCLASSES] = {0, 0, 0, 0};   // persistent grid of each recompression class
   int* d_retry = nullptr;              // per slot: tiles redone with the wider sketch
@@ -270,7 +270,7 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
     const int64_t npm = c->npairs_max > 0 ? c->npairs_max : 1;
     c->wl_cap = npm * max_batch;
     ACHK(c->alloc(&c->wl, (size_t)RCLASSES * c->wl_cap));
-    ACHK(c->alloc(&c->wl_count, (size_t)RCLASSES * c->nB));
+    ACHK(c->alloc(&c->wl_count, (size_t)2 * RCLASSES * c->nB));
   }
   c->slots.resize(max_batch);
   for (int s = 0; s < max_batch; ++s) {
@@ -512,7 +512,7 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
   c->ev_used = 0;
   c->timed.clear();
   HIPCHK(launch_fill_int(c->d_retry, c->max_batch, 0, st));
-  HIPCHK(launch_fill_int(c->wl_count, (int64_t)RCLASSES * c->nB, 0, st));
+  HIPCHK(launch_fill_int(c->wl_count, (int64_t)2 * RCLASSES * c->nB, 0, st));
   // per-slot thresholds of the recompression (the adaptive budget depends on theta); the previous call
   // ended with a synchronisation, so the pinned staging buffer is free
   const double bmax_ = (double)(c->n < c->b ? c->n : c->b);

@@ -44,7 +44,7 @@ struct SlotArgs {
   int64_t scr_stride;   // doubles of recompression scratch per resident work group
   double* scr_pool;     // scratch of the running recompression class (shared by all slots)
   int2* wl;             // recompression work lists: RCLASSES lists of wl_cap (pair, slot) entries
-  int* wl_count;        // nB x RCLASSES entry counts (zeroed at the start of every evaluation)
+  int* wl_count;        // nB x RCLASSES x (heavy, light) entry counts (zeroed at the start of every evaluation)
   int64_t wl_cap;
   int rule_fixed;       // 1: fixed relative tolerance, 0: adaptive absolute budget
   int mixed;            // 1: adaptive budget plus per-pair storage precision (rounded factors)
