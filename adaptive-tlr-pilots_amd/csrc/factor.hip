@@ -418,8 +418,10 @@ __device__ __forceinline__ int recomp_class(int r2) { return r2 <= 8 ? 0 : (r2 <
 // Work lists of step k: one thread per (pair, slot); pairs with r_ik, r_jk > 0 go to the list of their
 // class, the heavy ones (wide cores: r_ij + r_jk above the class's threshold) from the front and the rest
 // from the back, so that the persistent grid starts the long pairs first.  Ranks outside [0, rank_cap]
-// trip the guard instead.  The order inside a list depends on the atomics; every pair is computed
-// independently of the others, so results do not.
+// trip the guard instead.  Pairs (i, k+1), which update the next panel column, go to separate panel lists
+// so that they can run before the rest and let the next step's POTRF and TRSM start early.  The order
+// inside a list depends on the atomics; every pair is computed independently of the others, so results
+// do not.
 __global__ __launch_bounds__(256) void k_recomp_list(SlotArgs sa, int k, int npairs) {
   const int q = blockIdx.x * 256 + threadIdx.x;
   const int slot = blockIdx.y;
@@ -437,10 +439,12 @@ __global__ __launch_bounds__(256) void k_recomp_list(SlotArgs sa, int k, int npa
   if (rik == 0 || r2 == 0) return;
   const int c = recomp_class(r2);
   const int heavy_w[RCLASSES] = {16, 32, 48, 80};
-  int* cnt = sa.wl_count + 2 * (RCLASSES * k + c);
-  int2* list = sa.wl + (int64_t)c * sa.wl_cap;
+  const bool panel = j == k + 1;
+  int* cnt = (panel ? sa.wlp_count : sa.wl_count) + 2 * (RCLASSES * k + c);
+  const int64_t cap = panel ? sa.wlp_cap : sa.wl_cap;
+  int2* list = (panel ? sa.wlp : sa.wl) + (int64_t)c * cap;
   if (r1 + r2 > heavy_w[c]) list[atomicAdd(cnt, 1)] = make_int2(q, slot);
-  else list[sa.wl_cap - 1 - atomicAdd(cnt + 1, 1)] = make_int2(q, slot);
+  else list[cap - 1 - atomicAdd(cnt + 1, 1)] = make_int2(q, slot);
 }
 
 // One pair (i, j) = pair index q of step k, for slot `slot`; scr is the work group's scratch.
@@ -623,13 +627,16 @@ struct RecompWaves {
 };
 
 template <int RPL, int NW, int CPW>
-__global__ __launch_bounds__(64 * NW, (RecompWaves<NW, CPW>::value)) void k_recompress(SlotArgs sa, int k, int cls) {
-  const int nh = __builtin_amdgcn_readfirstlane(sa.wl_count[2 * (RCLASSES * k + cls)]);
-  const int cnt = nh + __builtin_amdgcn_readfirstlane(sa.wl_count[2 * (RCLASSES * k + cls) + 1]);
-  const int2* list = sa.wl + (int64_t)cls * sa.wl_cap;
+__global__ __launch_bounds__(64 * NW, (RecompWaves<NW, CPW>::value)) void k_recompress(SlotArgs sa, int k, int cls,
+                                                                                        int panel) {
+  const int* cnts = (panel ? sa.wlp_count : sa.wl_count) + 2 * (RCLASSES * k + cls);
+  const int64_t cap = panel ? sa.wlp_cap : sa.wl_cap;
+  const int nh = __builtin_amdgcn_readfirstlane(cnts[0]);
+  const int cnt = nh + __builtin_amdgcn_readfirstlane(cnts[1]);
+  const int2* list = (panel ? sa.wlp : sa.wl) + (int64_t)cls * cap;
   double* scr = sa.scr_pool + (int64_t)blockIdx.x * sa.scr_stride;
   for (int q = blockIdx.x; q < cnt; q += gridDim.x) {
-    const int2 e = list[q < nh ? q : sa.wl_cap - 1 - (q - nh)];
+    const int2 e = list[q < nh ? q : cap - 1 - (q - nh)];
     recompress_pair<RPL, NW, CPW>(sa, k, __builtin_amdgcn_readfirstlane(e.x), __builtin_amdgcn_readfirstlane(e.y),
                                   scr);
     __syncthreads();
@@ -695,11 +702,11 @@ static hipError_t recomp_attr() {
 }
 
 template <int RPL, int NW, int CPW>
-static hipError_t launch_recomp_t(const SlotArgs& sa, int k, int cls, int grid, hipStream_t s) {
+static hipError_t launch_recomp_t(const SlotArgs& sa, int k, int cls, int panel, int grid, hipStream_t s) {
   hipError_t e = recomp_attr<RPL, NW, CPW>();
   if (e != hipSuccess) return e;
   const size_t lds = (size_t)RecompLayout<NW, CPW>::TOTAL * sizeof(double);
-  hipLaunchKernelGGL((k_recompress<RPL, NW, CPW>), dim3(grid), dim3(64 * NW), lds, s, sa, k, cls);
+  hipLaunchKernelGGL((k_recompress<RPL, NW, CPW>), dim3(grid), dim3(64 * NW), lds, s, sa, k, cls, panel);
   return hipGetLastError();
 }
 
@@ -734,24 +741,25 @@ hipError_t launch_recomp_list(const SlotArgs& sa, int k, int nslot, hipStream_t 
   return hipGetLastError();
 }
 
-// One persistent launch of class cls for step k (grid work groups, each looping over the class list;
-// scr is the class's scratch pool, grid x scr_stride doubles).
-hipError_t launch_recomp_class(const SlotArgs& sa_in, int k, int cls, int grid, double* scr, hipStream_t s) {
+// One persistent launch of class cls for step k (grid work groups, each looping over the class list,
+// the panel list when panel != 0; scr is the class's scratch pool, grid x scr_stride doubles).
+hipError_t launch_recomp_class(const SlotArgs& sa_in, int k, int cls, int panel, int grid, double* scr,
+                               hipStream_t s) {
   SlotArgs sa = sa_in;
   sa.scr_pool = scr;
   if (sa.b <= 256) {
     switch (cls) {
-      case 0: return launch_recomp_t<4, 4, 2>(sa, k, 0, grid, s);
-      case 1: return launch_recomp_t<4, 4, 4>(sa, k, 1, grid, s);
-      case 2: return launch_recomp_t<4, 8, 4>(sa, k, 2, grid, s);
-      default: return launch_recomp_t<4, 8, 8>(sa, k, 3, grid, s);
+      case 0: return launch_recomp_t<4, 4, 2>(sa, k, 0, panel, grid, s);
+      case 1: return launch_recomp_t<4, 4, 4>(sa, k, 1, panel, grid, s);
+      case 2: return launch_recomp_t<4, 8, 4>(sa, k, 2, panel, grid, s);
+      default: return launch_recomp_t<4, 8, 8>(sa, k, 3, panel, grid, s);
     }
   }
   switch (cls) {
-    case 0: return launch_recomp_t<8, 4, 2>(sa, k, 0, grid, s);
-    case 1: return launch_recomp_t<8, 4, 4>(sa, k, 1, grid, s);
-    case 2: return launch_recomp_t<8, 8, 4>(sa, k, 2, grid, s);
-    default: return launch_recomp_t<8, 8, 8>(sa, k, 3, grid, s);
+    case 0: return launch_recomp_t<8, 4, 2>(sa, k, 0, panel, grid, s);
+    case 1: return launch_recomp_t<8, 4, 4>(sa, k, 1, panel, grid, s);
+    case 2: return launch_recomp_t<8, 8, 4>(sa, k, 2, panel, grid, s);
+    default: return launch_recomp_t<8, 8, 8>(sa, k, 3, panel, grid, s);
   }
 }
 

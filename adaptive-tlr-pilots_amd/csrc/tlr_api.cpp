@@ -73,9 +73,14 @@ struct tlr_ctx {
   double* scr_cls[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};   // the part of each class
   hipStream_t cls_stream[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};   // [0] = stream
   hipEvent_t ev_list = nullptr, ev_cls[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev_pan[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};   // side stream passed its panel pairs
   int2* wl = nullptr;                  // recompression work lists (RCLASSES x wl_cap)
   int* wl_count = nullptr;             // nB x RCLASSES x (heavy, light) list lengths
   int64_t wl_cap = 0;
+  int2* wlp = nullptr;                 // panel-pair lists (pairs (i, k+1))
+  int* wlp_count = nullptr;
+  int64_t wlp_cap = 0;
+  double* scr_pan[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};   // scratch of the panel launches
   int recomp_grid[RCLASSES] = {0, 0, 0, 0};   // persistent grid of each recompression class
   int* d_retry = nullptr;              // per slot: tiles redone with the wider sketch
   double* d_thr = nullptr;             // per slot (thr_rel, thr_abs) of the current batch
@@ -111,6 +116,7 @@ struct tlr_ctx {
     if (ev_list) (void)hipEventDestroy(ev_list);
     for (int q = 1; q < RCLASSES; ++q) {
       if (ev_cls[q]) (void)hipEventDestroy(ev_cls[q]);
+      if (ev_pan[q]) (void)hipEventDestroy(ev_pan[q]);
       if (cls_stream[q]) (void)hipStreamDestroy(cls_stream[q]);
     }
     if (stream) (void)hipStreamDestroy(stream);
@@ -150,6 +156,9 @@ struct tlr_ctx {
     a.wl = wl;
     a.wl_count = wl_count;
     a.wl_cap = wl_cap;
+    a.wlp = wlp;
+    a.wlp_count = wlp_count;
+    a.wlp_cap = wlp_cap;
     a.rule_fixed = 0;
     a.mixed = 0;
     a.thr = d_thr;
@@ -254,23 +263,30 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
       c->recomp_grid[q] = (per_cu[q] > 0 ? per_cu[q] : 1) * (cus > 0 ? cus : 1);
       gsum += c->recomp_grid[q];
     }
-    // the classes of one step run concurrently (one stream each), so each has its own scratch
-    ACHK(c->alloc(&c->scr_pool, (size_t)gsum * c->scr_stride));
+    // the classes of one step run concurrently (one stream each) and so do the panel launches, so each
+    // class has its own scratch twice over
+    ACHK(c->alloc(&c->scr_pool, (size_t)2 * gsum * c->scr_stride));
     int64_t off = 0;
     for (int q = 0; q < RCLASSES; ++q) {
       c->scr_cls[q] = c->scr_pool + off * c->scr_stride;
+      off += c->recomp_grid[q];
+      c->scr_pan[q] = c->scr_pool + off * c->scr_stride;
       off += c->recomp_grid[q];
     }
     c->cls_stream[0] = c->stream;
     for (int q = 1; q <= 2; ++q) {
       ACHK(hipStreamCreateWithFlags(&c->cls_stream[q], hipStreamNonBlocking));
       ACHK(hipEventCreateWithFlags(&c->ev_cls[q], hipEventDisableTiming));
+      ACHK(hipEventCreateWithFlags(&c->ev_pan[q], hipEventDisableTiming));
     }
     ACHK(hipEventCreateWithFlags(&c->ev_list, hipEventDisableTiming));
     const int64_t npm = c->npairs_max > 0 ? c->npairs_max : 1;
     c->wl_cap = npm * max_batch;
     ACHK(c->alloc(&c->wl, (size_t)RCLASSES * c->wl_cap));
     ACHK(c->alloc(&c->wl_count, (size_t)2 * RCLASSES * c->nB));
+    c->wlp_cap = (int64_t)(c->nB > 0 ? c->nB : 1) * max_batch;
+    ACHK(c->alloc(&c->wlp, (size_t)RCLASSES * c->wlp_cap));
+    ACHK(c->alloc(&c->wlp_count, (size_t)2 * RCLASSES * c->nB));
   }
   c->slots.resize(max_batch);
   for (int s = 0; s < max_batch; ++s) {
@@ -401,29 +417,43 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
 
 }  // extern "C"
 
-// Recompressions of step k: the work lists on the main stream, then the width classes at once on three
-// streams (the context has four hardware queues; the null stream takes one): the narrow class on the
-// main stream, 9-16 followed by 33-64 on a second, 17-32 on a third, joined back into the main stream.
-// The few wide pairs of a step take long; the narrow class fills the device meanwhile.
-static hipError_t launch_recompress_step(tlr_ctx* c, const SlotArgs& sa, int k, int nslot) {
-  if (c->nB - k - 1 < 2) return hipSuccess;
+// Step k after its TRSM and diagonal update, with one step of lookahead.  The work lists are built on the
+// main stream; each class runs on its stream (the context has four hardware queues and the null stream
+// takes one: the narrow class on the main stream, 17-32 on a second, 9-16 and 33-64 on a third), first
+// over its panel pairs (i, k+1), which update the next panel column, then over the rest.  Once the narrow
+// class is done and every stream has passed its panel pairs, the main stream runs POTRF, TRSM and the
+// diagonal update of step k+1 while the side streams finish their pairs; all are joined into the main
+// stream.  Nothing the lookahead writes (D, rhs, the (., k+1) factors, W_i and U_ik W_i of step k+1) is
+// touched by the pairs (i, j), j > k+1, that may still be running.
+static hipError_t launch_step_lookahead(tlr_ctx* c, const SlotArgs& sa, int k, int nslot) {
+  hipError_t e;
   TimedLaunch tt{nullptr, nullptr, -1, k, 0.0};
   const bool on = c->tic(5, tt);
-  hipError_t e = launch_recomp_list(sa, k, nslot, c->stream);
-  if (e != hipSuccess) return e;
+  if ((e = launch_recomp_list(sa, k, nslot, c->stream)) != hipSuccess) return e;
   if ((e = hipEventRecord(c->ev_list, c->stream)) != hipSuccess) return e;
   static const int plan[2][2] = {{2, -1}, {1, 3}};   // classes run by side streams 1 and 2, in order
   for (int q = 1; q <= 2; ++q) {
-    if ((e = hipStreamWaitEvent(c->cls_stream[q], c->ev_list, 0)) != hipSuccess) return e;
-    for (int u = 0; u < 2; ++u) {
-      const int cls = plan[q - 1][u];
-      if (cls < 0) continue;
-      if ((e = launch_recomp_class(sa, k, cls, c->recomp_grid[cls], c->scr_cls[cls], c->cls_stream[q])) != hipSuccess)
-        return e;
+    hipStream_t sq = c->cls_stream[q];
+    if ((e = hipStreamWaitEvent(sq, c->ev_list, 0)) != hipSuccess) return e;
+    for (int pass = 1; pass >= 0; --pass) {   // panel pairs first, then the rest
+      for (int u = 0; u < 2; ++u) {
+        const int cls = plan[q - 1][u];
+        if (cls < 0) continue;
+        if ((e = launch_recomp_class(sa, k, cls, pass, c->recomp_grid[cls], pass ? c->scr_pan[cls] : c->scr_cls[cls],
+                                     sq)) != hipSuccess)
+          return e;
+      }
+      if (pass == 1 && (e = hipEventRecord(c->ev_pan[q], sq)) != hipSuccess) return e;
     }
-    if ((e = hipEventRecord(c->ev_cls[q], c->cls_stream[q])) != hipSuccess) return e;
+    if ((e = hipEventRecord(c->ev_cls[q], sq)) != hipSuccess) return e;
   }
-  if ((e = launch_recomp_class(sa, k, 0, c->recomp_grid[0], c->scr_cls[0], c->stream)) != hipSuccess) return e;
+  if ((e = launch_recomp_class(sa, k, 0, 1, c->recomp_grid[0], c->scr_pan[0], c->stream)) != hipSuccess) return e;
+  if ((e = launch_recomp_class(sa, k, 0, 0, c->recomp_grid[0], c->scr_cls[0], c->stream)) != hipSuccess) return e;
+  for (int q = 1; q <= 2; ++q)
+    if ((e = hipStreamWaitEvent(c->stream, c->ev_pan[q], 0)) != hipSuccess) return e;
+  if ((e = launch_potrf(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
+  if ((e = launch_trsm(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
+  if ((e = launch_diag_update(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
   for (int q = 1; q <= 2; ++q)
     if ((e = hipStreamWaitEvent(c->stream, c->ev_cls[q], 0)) != hipSuccess) return e;
   c->toc(on, tt);
@@ -513,6 +543,7 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
   c->timed.clear();
   HIPCHK(launch_fill_int(c->d_retry, c->max_batch, 0, st));
   HIPCHK(launch_fill_int(c->wl_count, (int64_t)2 * RCLASSES * c->nB, 0, st));
+  HIPCHK(launch_fill_int(c->wlp_count, (int64_t)2 * RCLASSES * c->nB, 0, st));
   // per-slot thresholds of the recompression (the adaptive budget depends on theta); the previous call
   // ended with a synchronisation, so the pinned staging buffer is free
   const double bmax_ = (double)(c->n < c->b ? c->n : c->b);
@@ -555,11 +586,17 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
   SlotArgs sa = c->args();
   sa.rule_fixed = rule_fixed;
   sa.mixed = mixed;
-  for (int k = 0; k < c->nB; ++k) {
-    HIPCHK(launch_potrf(sa, k, m, st));
-    HIPCHK(launch_trsm(sa, k, m, st));
-    HIPCHK(launch_diag_update(sa, k, m, st));
-    HIPCHK(launch_recompress_step(c, sa, k, m));
+  HIPCHK(launch_potrf(sa, 0, m, st));
+  HIPCHK(launch_trsm(sa, 0, m, st));
+  HIPCHK(launch_diag_update(sa, 0, m, st));
+  for (int k = 0; k + 1 < c->nB; ++k) {
+    if (c->nB - k - 1 >= 2) {
+      HIPCHK(launch_step_lookahead(c, sa, k, m));   // recompressions of step k with POTRF ... of step k + 1
+    } else {
+      HIPCHK(launch_potrf(sa, k + 1, m, st));
+      HIPCHK(launch_trsm(sa, k + 1, m, st));
+      HIPCHK(launch_diag_update(sa, k + 1, m, st));
+    }
   }
   HIPCHK(launch_final(sa, m, st));
   HIPCHK(hipStreamSynchronize(st));

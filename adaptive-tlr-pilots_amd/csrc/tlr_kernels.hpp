@@ -46,6 +46,9 @@ struct SlotArgs {
   int2* wl;             // recompression work lists: RCLASSES lists of wl_cap (pair, slot) entries
   int* wl_count;        // nB x RCLASSES x (heavy, light) entry counts (zeroed at the start of every evaluation)
   int64_t wl_cap;
+  int2* wlp;            // the same for the pairs (i, k+1) that update the next panel column (run first)
+  int* wlp_count;
+  int64_t wlp_cap;
   int rule_fixed;       // 1: fixed relative tolerance, 0: adaptive absolute budget
   int mixed;            // 1: adaptive budget plus per-pair storage precision (rounded factors)
   const double* thr;    // per slot: (thr_rel, thr_abs)
@@ -84,7 +87,7 @@ hipError_t launch_potrf(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_trsm(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_recomp_list(const SlotArgs& sa, int k, int nslot, hipStream_t s);
-hipError_t launch_recomp_class(const SlotArgs& sa, int k, int cls, int grid, double* scr, hipStream_t s);
+hipError_t launch_recomp_class(const SlotArgs& sa, int k, int cls, int panel, int grid, double* scr, hipStream_t s);
 // resident work groups per CU of each recompression class (b selects the row-panel variant)
 hipError_t recompress_occupancy(int b, int blocks_per_cu[RCLASSES]);
 hipError_t launch_final(const SlotArgs& sa, int nslot, hipStream_t s);
