@@ -1,0 +1,48 @@
+"""Edge shapes of the tile grid against the oracle: a short last tile, tile sizes that are not multiples of
+the 32/64-row blocks of the kernels, a two-tile problem, and both rank rules; initial and final ranks
+identical, log L, log-det and quadratic form within 1e-6 * max(1, |v|) (the reference checker's tolerance)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # (N, b, theta, rule, value)
+    (2345, 250, (1.0, 0.3, 0.05), "adaptive", 0.9),
+    (1000, 300, (0.7, 0.15, 0.02), "adaptive", 0.9),
+    (777, 128, (1.5, 0.45, 0.08), "fixed", 1e-5),
+    (300, 250, (1.0, 0.2, 0.1), "adaptive", 0.5),
+    (1500, 500, (2.0, 0.6, 0.01), "fixed", 1e-4),
+]
+
+
+def _field(n, seed=11):
+    import tlr_oracle as O
+    rng = np.random.default_rng(seed)
+    S = rng.random((n, 2))
+    S = S[O.morton_order(S)]
+    y = np.linalg.cholesky(O.cov_full(S, np.array([1.0, 0.3, 0.05]))) @ rng.standard_normal(n)
+    return S, y
+
+
+def _close(a, b):
+    return abs(a - b) <= 1e-6 * max(1.0, abs(b))
+
+
+@pytest.mark.parametrize("n,b,theta,rule,value", CASES)
+def test_edge_shapes_vs_oracle(n, b, theta, rule, value):
+    import tlrgp
+    import tlr_oracle as O
+    S, y = _field(n)
+    bnd = tlrgp.tile_bounds(n, b)
+    th = np.array(theta)
+    print(f"case edge N={n} b={b} {rule}:{value} theta={theta}", flush=True)
+    ll, _, info = tlrgp.tlr_loglik(S, y, th, bnd, rule, value)
+    ll_o, _, info_o = O.tlr_loglik(S, y, th, bnd, rule, value)
+    np.testing.assert_array_equal(info["ranks_sigma"], info_o["ranks_sigma"])
+    assert info["completed"] == info_o["completed"]
+    if info_o["completed"]:
+        np.testing.assert_array_equal(info["ranks_factor"], info_o["ranks_factor"])
+        assert _close(ll, ll_o), (ll, ll_o)
+        assert _close(info["logdet"], info_o["logdet"]) and _close(info["quad"], info_o["quad"])
+    else:
+        assert info["failed_at"] == info_o["failed_at"]
