@@ -173,18 +173,79 @@ __global__ __launch_bounds__(512) void k_potrf(SlotArgs sa, int k) {
   if (tid == 0) P.logdet_part[k] = logacc;
 }
 
+// ----------------------------------------------------------------------------------------- TRTRI
+// Explicit inverse of L_kk (lower, b x b) into the slot's Linv, one work group per 32-column block q:
+// L X = E_q (the identity's columns of block q) by forward substitution over the 32-row panels p >= q,
+// X_p = Dinv_p (E_q,p - sum_{q <= t < p} L_pt X_t), with the inverted diagonal blocks from POTRF.  Rows
+// above block q are zero and are not written (Linv is only read on and below the diagonal).
+__global__ __launch_bounds__(256) void k_trtri(SlotArgs sa, int k) {
+  const int slot = blockIdx.z;
+  const SlotPtrs P = sa.slot(slot);
+  if (failed(P.status)) return;
+  const int b = sa.b;
+  const int c0 = NB * blockIdx.x;
+  const double* L = P.D + (int64_t)k * b * b;
+  const double* Dv = P.Dinv + (int64_t)k * sa.npan * NB * NB;
+  double* X = P.Linv + (int64_t)c0 * b;
+  const int cw = min(NB, b - c0);
+  __shared__ double Lb[NB][NB + 1];
+  __shared__ double Xb[NB][NB + 1];
+  __shared__ double tmp[NB][NB + 1];
+  __shared__ double Dl[NB][NB + 1];
+  const int tid = threadIdx.x;
+  // thread -> (row ll, columns cg, cg + 8, cg + 16, cg + 24)
+  const int ll = tid % NB, cg = tid / NB;
+  for (int p0 = c0; p0 < b; p0 += NB) {
+    const int wd = min(NB, b - p0);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q0 = c0; q0 < p0; q0 += NB) {
+      for (int e = tid; e < NB * NB; e += 256) {
+        const int rr = e % NB, qq = e / NB;
+        Lb[rr][qq] = (rr < wd) ? L[(p0 + rr) + (int64_t)(q0 + qq) * b] : 0.0;
+        Xb[rr][qq] = (qq < cw) ? X[(q0 + rr) + (int64_t)qq * b] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int qq = 0; qq < NB; ++qq) {
+        const double lv = Lb[ll][qq];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = fma(lv, Xb[qq][cg + 8 * u], acc[u]);
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < NB * NB; e += 256) Dl[e % NB][e / NB] = Dv[(p0 / NB) * NB * NB + e];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = cg + 8 * u;
+      tmp[ll][c] = (ll < wd) ? ((p0 + ll == c0 + c) ? 1.0 : 0.0) - acc[u] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = cg + 8 * u;
+      if (c < cw && ll < wd) {
+        double s = 0.0;
+        for (int m = 0; m <= ll; ++m) s = fma(Dl[ll][m], tmp[m][c], s);
+        X[(p0 + ll) + (int64_t)c * b] = s;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------------------------------ TRSM
 // Item 0: rhs_k -> z_k = L_kk^{-1} rhs_k.  Item m >= 1: tile i = k + m, V_ik <- L_kk^{-1} V_ik, then
-// W_i = V_ik^T V_ik (r x r) and X_i = U_ik W_i (b x r) for the diagonal update.  Blocked forward
-// substitution over the 32-row panels of L_kk using the stored inverses of its diagonal blocks.
+// W_i = V_ik^T V_ik (r x r) and X_i = U_ik W_i (b x r) for the diagonal update.  The solve is the product
+// with the explicit inverse from k_trtri, 8 columns at a time staged through LDS; thread t forms rows t
+// and brows - 1 - t so that every thread runs the same number of inner steps.
+constexpr int TRSM_LDS = 6144;   // doubles: 512 rows x 8 columns, or W_i (64 x 64) + 32 rows of U_ik
 __global__ __launch_bounds__(256) void k_trsm(SlotArgs sa, int k) {
   const int slot = blockIdx.z;
   const SlotPtrs P = sa.slot(slot);
   if (failed(P.status)) return;
   const int b = sa.b, rc = sa.rank_cap;
   const int item = blockIdx.x;
-  const double* L = P.D + (int64_t)k * b * b;
-  const double* Dv = P.Dinv + (int64_t)k * sa.npan * NB * NB;
+  const double* Li = P.Linv;
   double* X;
   int r;
   int64_t tik = -1;
@@ -203,55 +264,38 @@ __global__ __launch_bounds__(256) void k_trsm(SlotArgs sa, int k) {
   }
   const int bk = (int)min<int64_t>(b, sa.n - (int64_t)k * b);
   const int brows = (item == 0) ? bk : b;   // rhs holds exactly the tile's rows
-  __shared__ double Lb[NB][NB + 1];         // L[p0 + ll][q0 + qq]
-  __shared__ double XT[2][NB][64];          // Xb: X[q0 + qq][c]; tmp: corrected rows of the panel
-  double (*Xb)[64] = XT[0];
-  double (*tmp)[64] = XT[1];
-  __shared__ double Dl[NB][NB + 1];
+  __shared__ __align__(16) double Xs[TRSM_LDS];
   const int tid = threadIdx.x;
-  // thread -> (row ll, columns c = cg, cg + 4, ...): 32 rows x 8 column groups
-  const int ll = tid % NB, cg = tid / NB;
-  for (int p0 = 0; p0 < brows; p0 += NB) {
-    const int wd = min(NB, brows - p0);
-    double acc[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) acc[u] = 0.0;
-    // X[p0:p0+32] -= L[p0:p0+32, 0:p0] X[0:p0], 32 x 32 blocks of L staged through LDS
-    for (int q0 = 0; q0 < p0; q0 += NB) {
-      for (int e = tid; e < NB * NB; e += 256) {
-        const int rr = e % NB, qq = e / NB;
-        Lb[rr][qq] = (rr < wd) ? L[(p0 + rr) + (int64_t)(q0 + qq) * b] : 0.0;
-      }
-      for (int e = tid; e < NB * r; e += 256) {
-        const int qq = e % NB, c = e / NB;
-        Xb[qq][c] = X[(q0 + qq) + (int64_t)c * b];
-      }
-      __syncthreads();
-#pragma unroll 4
-      for (int qq = 0; qq < NB; ++qq) {
-        const double lv = Lb[ll][qq];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int c = cg + 8 * u;
-          if (c < r) acc[u] = fma(lv, Xb[qq][c], acc[u]);
-        }
-      }
-      __syncthreads();
-    }
-    for (int e = tid; e < NB * NB; e += 256) Dl[e % NB][e / NB] = Dv[(p0 / NB) * NB * NB + e];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int c = cg + 8 * u;
-      if (c < r) tmp[ll][c] = (ll < wd) ? X[(p0 + ll) + (int64_t)c * b] - acc[u] : 0.0;
+  const int half = (brows + 1) / 2;
+  for (int cb = 0; cb < r; cb += 8) {
+    const int cw = min(8, r - cb);
+    for (int e = tid; e < brows * 8; e += 256) {
+      const int q = e >> 3, c = e & 7;
+      Xs[e] = (c < cw) ? X[q + (int64_t)(cb + c) * b] : 0.0;
     }
     __syncthreads();
+    if (tid < half) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int c = cg + 8 * u;
-      if (c < r && ll < wd) {
-        double s = 0.0;
-        for (int m = 0; m <= ll; ++m) s = fma(Dl[ll][m], tmp[m][c], s);
-        X[(p0 + ll) + (int64_t)c * b] = s;
+      for (int h = 0; h < 2; ++h) {
+        const int p = h ? brows - 1 - tid : tid;
+        if (h && p == tid) break;
+        double acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] = 0.0;
+        const double* lrow = Li + p;
+        for (int q = 0; q <= p; ++q) {
+          const double lv = lrow[(int64_t)q * b];
+          const double2* xq = reinterpret_cast<const double2*>(Xs + 8 * q);
+#pragma unroll
+          for (int c2 = 0; c2 < 4; ++c2) {
+            const double2 xv = xq[c2];
+            acc[2 * c2] = fma(lv, xv.x, acc[2 * c2]);
+            acc[2 * c2 + 1] = fma(lv, xv.y, acc[2 * c2 + 1]);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          if (c < cw) X[p + (int64_t)(cb + c) * b] = acc[c];
       }
     }
     __syncthreads();
@@ -261,7 +305,8 @@ __global__ __launch_bounds__(256) void k_trsm(SlotArgs sa, int k) {
   double* Wi = P.Wd + (int64_t)(k + item) * rc * rc;
   double* Xi = P.Xd + (int64_t)(k + item) * b * rc;
   double* U = P.U + tik * sa.fstride;
-  __shared__ double Wsh[64 * 64];
+  double* Wsh = Xs;                  // r x r
+  double* Ub = Xs + 64 * 64;         // 32 rows x r
   const int w = wave_id(), l = lane_id();
   if (sa.mixed) {
     // mixed rule: storage precision of L_ik under the full budget (17:183-184), before W_i and X_i use it;
@@ -280,18 +325,17 @@ __global__ __launch_bounds__(256) void k_trsm(SlotArgs sa, int k) {
     if (l == 0) { Wsh[a + c * r] = s; Wsh[c + a * r] = s; Wi[a + c * r] = s; Wi[c + a * r] = s; }
   }
   __syncthreads();
-  // X_i = U W, rows staged through LDS 64 at a time (Xb and tmp form one 64 x 64 buffer)
-  double* Ub = &Xb[0][0];
-  for (int r0 = 0; r0 < b; r0 += 64) {
-    for (int e = tid; e < 64 * r; e += 256) {
-      const int rr = e % 64, a = e / 64, p = r0 + rr;
-      Ub[a * 64 + rr] = (p < b) ? U[p + (int64_t)a * b] : 0.0;
+  // X_i = U W, rows staged through LDS 32 at a time
+  for (int r0 = 0; r0 < b; r0 += 32) {
+    for (int e = tid; e < 32 * r; e += 256) {
+      const int rr = e % 32, a = e / 32, p = r0 + rr;
+      Ub[a * 32 + rr] = (p < b) ? U[p + (int64_t)a * b] : 0.0;
     }
     __syncthreads();
-    for (int e = tid; e < 64 * r; e += 256) {
-      const int rr = e % 64, c = e / 64, p = r0 + rr;
+    for (int e = tid; e < 32 * r; e += 256) {
+      const int rr = e % 32, c = e / 32, p = r0 + rr;
       double s = 0.0;
-      for (int a = 0; a < r; ++a) s = fma(Ub[a * 64 + rr], Wsh[a + c * r], s);
+      for (int a = 0; a < r; ++a) s = fma(Ub[a * 32 + rr], Wsh[a + c * r], s);
       if (p < b) Xi[p + (int64_t)c * b] = s;
     }
     __syncthreads();
@@ -323,8 +367,9 @@ __global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
   while (I * (I + 1) / 2 > e) --I;
   const int J = e - I * (I + 1) / 2;
   const int pb = 64 * I, qb = 64 * J;
-  __shared__ double Xs[64][64];
-  __shared__ double Us[64][64];
+  // columns staged 16 at a time (16 KB of LDS, so that several work groups share a CU)
+  __shared__ double Xs[16][64];
+  __shared__ double Us[16][64];
   __shared__ double zv[64];
   const int tr = tid / 16, tc = tid % 16;
   double acc[4][4];
@@ -332,22 +377,26 @@ __global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
-  for (int e2 = tid; e2 < 64 * r; e2 += 256) {
-    const int rr = e2 % 64, c = e2 / 64;
-    Xs[c][rr] = (pb + rr < b) ? Xi[pb + rr + (int64_t)c * b] : 0.0;
-    Us[c][rr] = (qb + rr < b) ? U[qb + rr + (int64_t)c * b] : 0.0;
-  }
-  __syncthreads();
-  for (int c = 0; c < r; ++c) {
-    double xv[4], uv[4];
+  for (int c0 = 0; c0 < r; c0 += 16) {
+    const int cn = min(16, r - c0);
+    if (c0 > 0) __syncthreads();
+    for (int e2 = tid; e2 < 64 * cn; e2 += 256) {
+      const int rr = e2 % 64, c = e2 / 64;
+      Xs[c][rr] = (pb + rr < b) ? Xi[pb + rr + (int64_t)(c0 + c) * b] : 0.0;
+      Us[c][rr] = (qb + rr < b) ? U[qb + rr + (int64_t)(c0 + c) * b] : 0.0;
+    }
+    __syncthreads();
+    for (int c = 0; c < cn; ++c) {
+      double xv[4], uv[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) xv[a] = Xs[c][tr * 4 + a];
+      for (int a = 0; a < 4; ++a) xv[a] = Xs[c][tr * 4 + a];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) uv[q] = Us[c][tc * 4 + q];
+      for (int q = 0; q < 4; ++q) uv[q] = Us[c][tc * 4 + q];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[a][q] = fma(xv[a], uv[q], acc[a][q]);
+        for (int q = 0; q < 4; ++q) acc[a][q] = fma(xv[a], uv[q], acc[a][q]);
+    }
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -416,7 +465,7 @@ struct RecompLayout {
 __device__ __forceinline__ int recomp_class(int r2) { return r2 <= 8 ? 0 : (r2 <= 16 ? 1 : (r2 <= 32 ? 2 : 3)); }
 
 // Work lists of step k: one thread per (pair, slot); pairs with r_ik, r_jk > 0 go to the list of their
-// class, the heavy ones (wide cores: r_ij + r_jk above the class's threshold) from the front and the rest
+// class (by r_jk, and by the core width r_ij + r_jk for the LDS), the heavy ones (wide cores: r_ij + r_jk above the class's threshold) from the front and the rest
 // from the back, so that the persistent grid starts the long pairs first.  Ranks outside [0, rank_cap]
 // trip the guard instead.  Pairs (i, k+1), which update the next panel column, go to separate panel lists
 // so that they can run before the rest and let the next step's POTRF and TRSM start early.  The order
@@ -437,10 +486,12 @@ __global__ __launch_bounds__(256) void k_recomp_list(SlotArgs sa, int k, int npa
     return;
   }
   if (rik == 0 || r2 == 0) return;
-  const int c = recomp_class(r2);
+  // the 4-wave classes keep cores up to 32 x 32 in LDS, the 8-wave ones up to 64 x 64: a wider core goes
+  // to an 8-wave class, whose panel (32 columns) also holds its r_jk <= 16 new columns
+  const int c = max(recomp_class(r2), r1 + r2 > 32 ? 2 : 0);
   const int heavy_w[RCLASSES] = {16, 32, 48, 80};
   const bool panel = j == k + 1;
-  int* cnt = (panel ? sa.wlp_count : sa.wl_count) + 2 * (RCLASSES * k + c);
+  int* cnt = (panel ? sa.wlp_count : sa.wl_count) + WL_STRIDE * (RCLASSES * k + c);
   const int64_t cap = panel ? sa.wlp_cap : sa.wl_cap;
   int2* list = (panel ? sa.wlp : sa.wl) + (int64_t)c * cap;
   if (r1 + r2 > heavy_w[c]) list[atomicAdd(cnt, 1)] = make_int2(q, slot);
@@ -537,7 +588,7 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
     G[a + c * w] = s;
   }
   __syncthreads();
-  const int sweeps = jacobi_svd<L::JTP>(G, w, J, w, w, sig, perm, &flag, 60);
+  const int sweeps = jacobi_svd_auto<L::JTP>(G, w, J, w, w, sig, perm, &flag, 60);
   if (sweeps == -2) {   // non-finite core: report, keep the tile unchanged
     if (tid == 0) atomicMax(&P.status[ST_GUARD], 2);
     return;
@@ -626,16 +677,24 @@ struct RecompWaves {
   static constexpr int value = NW * CPW <= 8 ? 3 : (NW * CPW <= 16 ? 2 : 1);
 };
 
+// Entries are claimed one at a time from the list's claim counter, in list order (heavy pairs first), so a
+// work group that becomes resident late (its CU still held by another kernel) only takes what is left.
 template <int RPL, int NW, int CPW>
 __global__ __launch_bounds__(64 * NW, (RecompWaves<NW, CPW>::value)) void k_recompress(SlotArgs sa, int k, int cls,
                                                                                         int panel) {
-  const int* cnts = (panel ? sa.wlp_count : sa.wl_count) + 2 * (RCLASSES * k + cls);
+  int* cnts = (panel ? sa.wlp_count : sa.wl_count) + WL_STRIDE * (RCLASSES * k + cls);
   const int64_t cap = panel ? sa.wlp_cap : sa.wl_cap;
   const int nh = __builtin_amdgcn_readfirstlane(cnts[0]);
   const int cnt = nh + __builtin_amdgcn_readfirstlane(cnts[1]);
   const int2* list = (panel ? sa.wlp : sa.wl) + (int64_t)cls * cap;
   double* scr = sa.scr_pool + (int64_t)blockIdx.x * sa.scr_stride;
-  for (int q = blockIdx.x; q < cnt; q += gridDim.x) {
+  __shared__ int s_claim;
+  for (;;) {
+    if (threadIdx.x == 0) s_claim = atomicAdd(&cnts[2], 1);
+    __syncthreads();
+    const int q = __builtin_amdgcn_readfirstlane(s_claim);
+    __syncthreads();
+    if (q >= cnt) break;
     const int2 e = list[q < nh ? q : cap - 1 - (q - nh)];
     recompress_pair<RPL, NW, CPW>(sa, k, __builtin_amdgcn_readfirstlane(e.x), __builtin_amdgcn_readfirstlane(e.y),
                                   scr);
@@ -673,6 +732,11 @@ __global__ __launch_bounds__(256) void k_final(SlotArgs sa) {
 // ------------------------------------------------------------------------------------ launchers
 hipError_t launch_potrf(const SlotArgs& sa, int k, int nslot, hipStream_t s) {
   hipLaunchKernelGGL(k_potrf, dim3(1, 1, nslot), dim3(512), 0, s, sa, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_trtri(const SlotArgs& sa, int k, int nslot, hipStream_t s) {
+  hipLaunchKernelGGL(k_trtri, dim3(sa.npan, 1, nslot), dim3(256), 0, s, sa, k);
   return hipGetLastError();
 }
 

@@ -261,6 +261,145 @@ __device__ __forceinline__ void panel_axpy(RegPanel<RPL, NW, CPW>& P, const doub
   }
 }
 
+// Orders the LDS accesses of one wave: the stores before it are seen by every lane's loads after it (a
+// wave's LDS operations execute in issue order; the fences keep the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A double moved between lanes by a DPP pattern within rows of 16 lanes (two 32-bit moves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(x & 0xffffffffll), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over aligned groups of TP (2, 4, 8 or 16) lanes; every lane of a group gets the group's sum.  The
+// exchanges are DPP moves (quad permutations, then mirrors within 8 and 16 lanes), with no LDS round trip.
+template <int TP>
+__device__ __forceinline__ double group_sum_dpp(double v) {
+  static_assert(TP == 2 || TP == 4 || TP == 8 || TP == 16, "group of 2, 4, 8 or 16 lanes");
+  v += dpp_move<0xB1>(v);                    // quad_perm(1, 0, 3, 2)
+  if (TP >= 4) v += dpp_move<0x4E>(v);       // quad_perm(2, 3, 0, 1)
+  if (TP >= 8) v += dpp_move<0x141>(v);      // row_half_mirror: lane l <-> 7 - l within 8
+  if (TP >= 16) v += dpp_move<0x140>(v);     // row_mirror: lane l <-> 15 - l within 16
+  return v;
+}
+
+// Rotation of the column pair (p, q) in one-sided Jacobi: given al = ||g_p||^2, be = ||g_q||^2 and
+// ga = g_p . g_q, returns false when the pair is already orthogonal to the tolerance (|ga| <= tol
+// sqrt(al be)) or either column is below the noise floor, else the cosine and sine of the rotation that
+// makes the columns orthogonal: t = tan(angle) = sign(d) h / (|d| + sqrt(d^2 + h^2)) with d = be - al and
+// h = 2 ga (t = 1 when d = 0), cs = 1 / sqrt(1 + t^2), sn = cs t.
+__device__ __forceinline__ bool jacobi_rotation(double al, double be, double ga, double floor2, double tol,
+                                                double* cs, double* sn) {
+  if (!(ga != 0.0 && al > floor2 && be > floor2 && ga * ga > tol * tol * al * be)) return false;
+  const double d = be - al, h = 2.0 * ga;
+  const double t = d == 0.0 ? 1.0 : (d > 0.0 ? h : -h) / (fabs(d) + sqrt(fma(d, d, h * h)));
+  const double c = 1.0 / sqrt(fma(t, t, 1.0));
+  *cs = c;
+  *sn = c * t;
+  return true;
+}
+
+// jacobi_svd for n <= 128 / TP (TP = 4, 8 or 16 lanes per column pair), run by one wave alone: its 64
+// lanes hold every column pair of a round (rows tl, tl + TP, ... of both columns and of J in registers), so
+// rounds are separated by a wave-level fence instead of a work-group barrier, each column is read once per
+// round, and the group sums are DPP exchanges.  Same ordering, rotation, noise floor and stopping rule as
+// jacobi_svd.  G, J, sig and perm must be in LDS (the wave-level fence orders LDS accesses only).  Only one
+// wave may call it; the caller synchronises the work group after.
+template <int TP>
+__device__ int jacobi_svd_w0(double* G, int ldg, double* J, int ldj, int n, double* sig, int* perm,
+                             int max_sweeps) {
+  constexpr int UR = 128 / (TP * TP) > 0 ? 128 / (TP * TP) : 1;   // rows per lane
+  const int l = lane_id();
+  for (int e = l; e < n * n; e += 64) {
+    const int r = e % n, c = e / n;
+    J[r + c * ldj] = (r == c) ? 1.0 : 0.0;
+  }
+  double cn = 0.0;
+  if (l < n)
+    for (int i = 0; i < n; ++i) { const double g = G[i + l * ldg]; cn = fma(g, g, cn); }
+  const double floor2 = 4.930380657631324e-32 * wave_sum(cn);   // eps^2 ||G||_F^2
+  const int np = (n + 1) & ~1, npairs = np / 2, m = np - 1;
+  const int pi = l / TP, tl = l % TP;
+  const double tol = 2.220446049250313e-16 * sqrt((double)n);
+  int sweeps = -1;
+  wave_lds_sync();
+  for (int s = 0; s < max_sweeps && n > 1; ++s) {
+    bool rotated = false;
+    // round r pairs (r + pi) mod m with (r - pi) mod m (pair 0: r with m); both advance by one per round
+    int pp = pi % m, qq = pi <= m ? (m - pi) % m : 0;
+    for (int r = 0; r < m; ++r) {
+      int p = n, q = n;
+      if (pi < npairs) {
+        if (pi == 0) { p = r; q = m; } else { p = pp; q = qq; }
+        if (p > q) { const int x = p; p = q; q = x; }
+      }
+      pp = pp + 1 == m ? 0 : pp + 1;
+      qq = qq + 1 == m ? 0 : qq + 1;
+      if (q < n) {
+        double gp[UR], gq[UR], jp[UR], jq[UR];
+#pragma unroll
+        for (int u = 0; u < UR; ++u) {
+          const int i = tl + TP * u;
+          gp[u] = i < n ? G[i + p * ldg] : 0.0;
+          gq[u] = i < n ? G[i + q * ldg] : 0.0;
+          jp[u] = i < n ? J[i + p * ldj] : 0.0;
+          jq[u] = i < n ? J[i + q * ldj] : 0.0;
+        }
+        double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+        for (int u = 0; u < UR; ++u) {
+          al = fma(gp[u], gp[u], al); be = fma(gq[u], gq[u], be); ga = fma(gp[u], gq[u], ga);
+        }
+        al = group_sum_dpp<TP>(al); be = group_sum_dpp<TP>(be); ga = group_sum_dpp<TP>(ga);
+        double cs, sn;
+        if (jacobi_rotation(al, be, ga, floor2, tol, &cs, &sn)) {
+#pragma unroll
+          for (int u = 0; u < UR; ++u) {
+            const int i = tl + TP * u;
+            if (i < n) {
+              G[i + p * ldg] = cs * gp[u] - sn * gq[u];
+              G[i + q * ldg] = sn * gp[u] + cs * gq[u];
+              J[i + p * ldj] = cs * jp[u] - sn * jq[u];
+              J[i + q * ldj] = sn * jp[u] + cs * jq[u];
+            }
+          }
+          rotated = true;
+        }
+      }
+      wave_lds_sync();
+    }
+    if (!__any(rotated)) { sweeps = s + 1; break; }
+  }
+  if (n <= 1) sweeps = 0;
+  if (l < n) {
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) { const double g = G[i + l * ldg]; acc += g * g; }
+    const double sv = sqrt(acc);
+    sig[l] = (sv == sv && sv <= 1.7976931348623157e308) ? sv : -1.0;
+  }
+  wave_lds_sync();
+  bool bad = false;
+  if (l < n) {
+    const double sj = sig[l];
+    int rank = 0;
+    for (int i = 0; i < n; ++i) {
+      const double si = sig[i];
+      rank += (si > sj) || (si == sj && i < l);
+    }
+    perm[rank] = l;
+    bad = sj < 0.0;
+  }
+  wave_lds_sync();
+  return __any(bad) ? -2 : sweeps;
+}
+
 // One-sided Jacobi SVD of the n x n matrix G (column-major, ldg; n <= 2*(NT/TP)), rotations
 // accumulated into J (initialised to I here).  On return the columns of G are mutually orthogonal,
 // G_out = G_in J, sig[j] = ||G_out[:, j]|| and perm[0..n) lists columns by descending sig (ties by
@@ -312,11 +451,9 @@ __device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double*
             const double gp = G[i + p * ldg], gq = G[i + q * ldg];
             al += gp * gp; be += gq * gq; ga += gp * gq;
           }
-          al = group_sum<TP>(al); be = group_sum<TP>(be); ga = group_sum<TP>(ga);
-          if (ga != 0.0 && al > floor2 && be > floor2 && fabs(ga) > tol * sqrt(al) * sqrt(be)) {
-            const double zeta = (be - al) / (2.0 * ga);
-            const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-            const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+          al = group_sum_dpp<TP>(al); be = group_sum_dpp<TP>(be); ga = group_sum_dpp<TP>(ga);
+          double cs, sn;
+          if (jacobi_rotation(al, be, ga, floor2, tol, &cs, &sn)) {
             for (int i = tl; i < n; i += TP) {
               const double gp = G[i + p * ldg], gq = G[i + q * ldg];
               G[i + p * ldg] = cs * gp - sn * gq;
@@ -358,6 +495,25 @@ __device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double*
   for (int j = 0; j < n; ++j)
     if (sig[j] < 0.0) return -2;
   return sweeps;
+}
+
+// jacobi_svd, through jacobi_svd_w0 (one wave; 16, 8 or 4 lanes per column pair) when n <= 32, in which case G, J,
+// sig and perm must be in LDS.  Every thread of the work group calls it; flag is an LDS int (it also
+// carries the wave-0 result to the other waves).
+template <int TP>
+__device__ int jacobi_svd_auto(double* G, int ldg, double* J, int ldj, int n, double* sig, int* perm, int* flag,
+                               int max_sweeps) {
+  if (n <= 32) {
+    if (wave_id() == 0) {
+      const int r = n <= 8 ? jacobi_svd_w0<16>(G, ldg, J, ldj, n, sig, perm, max_sweeps)
+                           : (n <= 16 ? jacobi_svd_w0<8>(G, ldg, J, ldj, n, sig, perm, max_sweeps)
+                                      : jacobi_svd_w0<4>(G, ldg, J, ldj, n, sig, perm, max_sweeps));
+      if (lane_id() == 0) *flag = r;
+    }
+    __syncthreads();
+    return *flag;
+  }
+  return jacobi_svd<TP>(G, ldg, J, ldj, n, sig, perm, flag, max_sweeps);
 }
 
 }  // namespace tlr

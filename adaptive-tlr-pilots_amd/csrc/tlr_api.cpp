@@ -75,7 +75,7 @@ struct tlr_ctx {
   hipEvent_t ev_list = nullptr, ev_cls[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t ev_pan[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};   // side stream passed its panel pairs
   int2* wl = nullptr;                  // recompression work lists (RCLASSES x wl_cap)
-  int* wl_count = nullptr;             // nB x RCLASSES x (heavy, light) list lengths
+  int* wl_count = nullptr;             // nB x RCLASSES x WL_STRIDE list counters
   int64_t wl_cap = 0;
   int2* wlp = nullptr;                 // panel-pair lists (pairs (i, k+1))
   int* wlp_count = nullptr;
@@ -283,16 +283,17 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
     const int64_t npm = c->npairs_max > 0 ? c->npairs_max : 1;
     c->wl_cap = npm * max_batch;
     ACHK(c->alloc(&c->wl, (size_t)RCLASSES * c->wl_cap));
-    ACHK(c->alloc(&c->wl_count, (size_t)2 * RCLASSES * c->nB));
+    ACHK(c->alloc(&c->wl_count, (size_t)WL_STRIDE * RCLASSES * c->nB));
     c->wlp_cap = (int64_t)(c->nB > 0 ? c->nB : 1) * max_batch;
     ACHK(c->alloc(&c->wlp, (size_t)RCLASSES * c->wlp_cap));
-    ACHK(c->alloc(&c->wlp_count, (size_t)2 * RCLASSES * c->nB));
+    ACHK(c->alloc(&c->wlp_count, (size_t)WL_STRIDE * RCLASSES * c->nB));
   }
   c->slots.resize(max_batch);
   for (int s = 0; s < max_batch; ++s) {
     SlotPtrs& p = c->slots[s].p;
     ACHK(c->alloc(&p.D, (size_t)c->nB * b * b));
     ACHK(c->alloc(&p.Dinv, (size_t)c->nB * c->npan * 32 * 32));
+    ACHK(c->alloc(&p.Linv, (size_t)b * b));
     ACHK(c->alloc(&p.U, (size_t)nt * b * RANK_CAP));
     ACHK(c->alloc(&p.V, (size_t)nt * b * RANK_CAP));
     ACHK(c->alloc(&p.rank, (size_t)nt));
@@ -419,19 +420,22 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
 
 // Step k after its TRSM and diagonal update, with one step of lookahead.  The work lists are built on the
 // main stream; each class runs on its stream (the context has four hardware queues and the null stream
-// takes one: the narrow class on the main stream, 17-32 on a second, 9-16 and 33-64 on a third), first
-// over its panel pairs (i, k+1), which update the next panel column, then over the rest.  Once the narrow
-// class is done and every stream has passed its panel pairs, the main stream runs POTRF, TRSM and the
-// diagonal update of step k+1 while the side streams finish their pairs; all are joined into the main
-// stream.  Nothing the lookahead writes (D, rhs, the (., k+1) factors, W_i and U_ik W_i of step k+1) is
-// touched by the pairs (i, j), j > k+1, that may still be running.
+// takes one: the narrow class on the main stream, 17-32 on a second, 33-64 and then 9-16 on a third, the
+// widest first so that the longest pairs start at once), first over its panel pairs (i, k+1),
+// which update the next panel column, then over the rest.  POTRF of step k+1 needs only D_{k+1}, final
+// after the diagonal update of step k, so the main stream runs it (and the explicit inverse of L_{k+1,k+1})
+// first, while the side streams start their pairs; TRSM and the diagonal update of step k+1 follow once
+// every stream has passed its panel pairs, and all
+// streams are joined into the main stream.  Nothing the lookahead writes (D, rhs, the (., k+1) factors, W_i
+// and U_ik W_i of step k+1) is touched by the pairs (i, j), j > k+1, that may still be running, and the
+// lists of step k are complete before POTRF can record a failure.
 static hipError_t launch_step_lookahead(tlr_ctx* c, const SlotArgs& sa, int k, int nslot) {
   hipError_t e;
   TimedLaunch tt{nullptr, nullptr, -1, k, 0.0};
   const bool on = c->tic(5, tt);
   if ((e = launch_recomp_list(sa, k, nslot, c->stream)) != hipSuccess) return e;
   if ((e = hipEventRecord(c->ev_list, c->stream)) != hipSuccess) return e;
-  static const int plan[2][2] = {{2, -1}, {1, 3}};   // classes run by side streams 1 and 2, in order
+  static const int plan[2][2] = {{2, -1}, {3, 1}};   // classes run by side streams 1 and 2, in order
   for (int q = 1; q <= 2; ++q) {
     hipStream_t sq = c->cls_stream[q];
     if ((e = hipStreamWaitEvent(sq, c->ev_list, 0)) != hipSuccess) return e;
@@ -447,11 +451,12 @@ static hipError_t launch_step_lookahead(tlr_ctx* c, const SlotArgs& sa, int k, i
     }
     if ((e = hipEventRecord(c->ev_cls[q], sq)) != hipSuccess) return e;
   }
+  if ((e = launch_potrf(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
+  if ((e = launch_trtri(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
   if ((e = launch_recomp_class(sa, k, 0, 1, c->recomp_grid[0], c->scr_pan[0], c->stream)) != hipSuccess) return e;
   if ((e = launch_recomp_class(sa, k, 0, 0, c->recomp_grid[0], c->scr_cls[0], c->stream)) != hipSuccess) return e;
   for (int q = 1; q <= 2; ++q)
     if ((e = hipStreamWaitEvent(c->stream, c->ev_pan[q], 0)) != hipSuccess) return e;
-  if ((e = launch_potrf(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
   if ((e = launch_trsm(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
   if ((e = launch_diag_update(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
   for (int q = 1; q <= 2; ++q)
@@ -542,8 +547,8 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
   c->ev_used = 0;
   c->timed.clear();
   HIPCHK(launch_fill_int(c->d_retry, c->max_batch, 0, st));
-  HIPCHK(launch_fill_int(c->wl_count, (int64_t)2 * RCLASSES * c->nB, 0, st));
-  HIPCHK(launch_fill_int(c->wlp_count, (int64_t)2 * RCLASSES * c->nB, 0, st));
+  HIPCHK(launch_fill_int(c->wl_count, (int64_t)WL_STRIDE * RCLASSES * c->nB, 0, st));
+  HIPCHK(launch_fill_int(c->wlp_count, (int64_t)WL_STRIDE * RCLASSES * c->nB, 0, st));
   // per-slot thresholds of the recompression (the adaptive budget depends on theta); the previous call
   // ended with a synchronisation, so the pinned staging buffer is free
   const double bmax_ = (double)(c->n < c->b ? c->n : c->b);
@@ -587,6 +592,7 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
   sa.rule_fixed = rule_fixed;
   sa.mixed = mixed;
   HIPCHK(launch_potrf(sa, 0, m, st));
+  HIPCHK(launch_trtri(sa, 0, m, st));
   HIPCHK(launch_trsm(sa, 0, m, st));
   HIPCHK(launch_diag_update(sa, 0, m, st));
   for (int k = 0; k + 1 < c->nB; ++k) {
@@ -594,6 +600,7 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
       HIPCHK(launch_step_lookahead(c, sa, k, m));   // recompressions of step k with POTRF ... of step k + 1
     } else {
       HIPCHK(launch_potrf(sa, k + 1, m, st));
+      HIPCHK(launch_trtri(sa, k + 1, m, st));
       HIPCHK(launch_trsm(sa, k + 1, m, st));
       HIPCHK(launch_diag_update(sa, k + 1, m, st));
     }

@@ -12,10 +12,13 @@ enum { ST_RANKCAP = 0, ST_FAILED = 1, ST_GUARD = 2, ST_COUNT = 4 };
 
 // width classes of the recompression, by r_jk: 1-8, 9-16, 17-32, 33-64
 constexpr int RCLASSES = 4;
+// counters per (step, class) of a work list: heavy entries, light entries, entries claimed, unused
+constexpr int WL_STRIDE = 4;
 
 // Device buffers of one slot (one theta being evaluated).  Layouts (all FP64 column-major):
 //   D      nB tiles of b x b: diagonal tiles, factored in place into L_kk (lower)
 //   Dinv   nB x npan blocks of 32 x 32: inverses of the diagonal blocks of L_kk
+//   Linv   b x b: the whole L_kk^{-1} of the step being factored (TRSM by product)
 //   U, V   ntile slots of b x rank_cap: off-diagonal factors, tile (i,j) at tri_index(i,j)
 //   rank   ntile ints (current rank), rank0 the ranks after the initial compression
 //   Wd, Xd nB x (rank_cap^2) and nB x (b x rank_cap): W_i = V_ik^T V_ik and U_ik W_i of step k
@@ -23,6 +26,7 @@ constexpr int RCLASSES = 4;
 struct SlotPtrs {
   double* D;
   double* Dinv;
+  double* Linv;       // b x b: L_kk^{-1} of the current step (lower)
   double* U;
   double* V;
   int* rank;
@@ -33,8 +37,8 @@ struct SlotPtrs {
   double* logdet_part;
   int* status;
   double* result;   // [loglik, logdet, quad]
-  int* rlog;
-  int* prec;          // storage precision (8, 16, 32, 64) of every factor tile, mixed rule          // rank chosen by every recompression, at rlog_off[k] + pair index
+  int* rlog;          // rank chosen by every recompression, at rlog_off[k] + pair index
+  int* prec;          // storage precision (8, 16, 32, 64) of every factor tile, mixed rule
 };
 
 struct SlotArgs {
@@ -44,7 +48,7 @@ struct SlotArgs {
   int64_t scr_stride;   // doubles of recompression scratch per resident work group
   double* scr_pool;     // scratch of the running recompression class (shared by all slots)
   int2* wl;             // recompression work lists: RCLASSES lists of wl_cap (pair, slot) entries
-  int* wl_count;        // nB x RCLASSES x (heavy, light) entry counts (zeroed at the start of every evaluation)
+  int* wl_count;        // nB x RCLASSES x WL_STRIDE counters (zeroed at the start of every evaluation)
   int64_t wl_cap;
   int2* wlp;            // the same for the pairs (i, k+1) that update the next panel column (run first)
   int* wlp_count;
@@ -84,6 +88,7 @@ hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t 
 
 // factor.hip
 hipError_t launch_potrf(const SlotArgs& sa, int k, int nslot, hipStream_t s);
+hipError_t launch_trtri(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_trsm(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_recomp_list(const SlotArgs& sa, int k, int nslot, hipStream_t s);
