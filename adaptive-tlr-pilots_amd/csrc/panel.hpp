@@ -306,7 +306,7 @@ __device__ __forceinline__ bool jacobi_rotation(double al, double be, double ga,
   return true;
 }
 
-// jacobi_svd for n <= 128 / TP (TP = 4, 8 or 16 lanes per column pair), run by one wave alone: its 64
+// jacobi_svd for n <= 128 / TP (TP = 2, 4, 8 or 16 lanes per column pair), run by one wave alone: its 64
 // lanes hold every column pair of a round (rows tl, tl + TP, ... of both columns and of J in registers), so
 // rounds are separated by a wave-level fence instead of a work-group barrier, each column is read once per
 // round, and the group sums are DPP exchanges.  Same ordering, rotation, noise floor and stopping rule as
@@ -316,6 +316,7 @@ template <int TP>
 __device__ int jacobi_svd_w0(double* G, int ldg, double* J, int ldj, int n, double* sig, int* perm,
                              int max_sweeps) {
   constexpr int UR = 128 / (TP * TP) > 0 ? 128 / (TP * TP) : 1;   // rows per lane
+  constexpr bool HOIST = UR <= 8;   // J's rows loaded with G's (registers permitting)
   const int l = lane_id();
   for (int e = l; e < n * n; e += 64) {
     const int r = e % n, c = e / n;
@@ -343,14 +344,16 @@ __device__ int jacobi_svd_w0(double* G, int ldg, double* J, int ldj, int n, doub
       pp = pp + 1 == m ? 0 : pp + 1;
       qq = qq + 1 == m ? 0 : qq + 1;
       if (q < n) {
-        double gp[UR], gq[UR], jp[UR], jq[UR];
+        double gp[UR], gq[UR], jp[HOIST ? UR : 1], jq[HOIST ? UR : 1];
 #pragma unroll
         for (int u = 0; u < UR; ++u) {
           const int i = tl + TP * u;
           gp[u] = i < n ? G[i + p * ldg] : 0.0;
           gq[u] = i < n ? G[i + q * ldg] : 0.0;
-          jp[u] = i < n ? J[i + p * ldj] : 0.0;
-          jq[u] = i < n ? J[i + q * ldj] : 0.0;
+          if (HOIST) {
+            jp[HOIST ? u : 0] = i < n ? J[i + p * ldj] : 0.0;
+            jq[HOIST ? u : 0] = i < n ? J[i + q * ldj] : 0.0;
+          }
         }
         double al = 0.0, be = 0.0, ga = 0.0;
 #pragma unroll
@@ -366,8 +369,10 @@ __device__ int jacobi_svd_w0(double* G, int ldg, double* J, int ldj, int n, doub
             if (i < n) {
               G[i + p * ldg] = cs * gp[u] - sn * gq[u];
               G[i + q * ldg] = sn * gp[u] + cs * gq[u];
-              J[i + p * ldj] = cs * jp[u] - sn * jq[u];
-              J[i + q * ldj] = sn * jp[u] + cs * jq[u];
+              const double a = HOIST ? jp[HOIST ? u : 0] : J[i + p * ldj];
+              const double c = HOIST ? jq[HOIST ? u : 0] : J[i + q * ldj];
+              J[i + p * ldj] = cs * a - sn * c;
+              J[i + q * ldj] = sn * a + cs * c;
             }
           }
           rotated = true;
@@ -497,21 +502,132 @@ __device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double*
   return sweeps;
 }
 
-// jacobi_svd, through jacobi_svd_w0 (one wave; 16, 8 or 4 lanes per column pair) when n <= 32, in which case G, J,
-// sig and perm must be in LDS.  Every thread of the work group calls it; flag is an LDS int (it also
-// carries the wave-0 result to the other waves).
+// jacobi_svd for n <= TP * UR with (n + 1) / 2 * TP <= blockDim.x (TP = 2, 4, 8 or 16 lanes per column
+// pair): each pair's rows tl, tl + TP, ... of both columns (and of J when UR <= 8) are read once per round
+// into registers, the group sums are DPP exchanges, and the work group synchronises once per round.  Same
+// ordering, rotation, noise floor and stopping rule as jacobi_svd.  Every thread of the work group calls it.
+template <int TP, int UR>
+__device__ int jacobi_svd_mw(double* G, int ldg, double* J, int ldj, int n, double* sig, int* perm,
+                             int max_sweeps) {
+  constexpr bool HOIST = UR <= 8;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int e = tid; e < n * n; e += nt) {
+    const int r = e % n, c = e / n;
+    J[r + c * ldj] = (r == c) ? 1.0 : 0.0;
+  }
+  __shared__ double jac_floor_mw;
+  for (int c = tid; c < n; c += nt) {
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) { const double g = G[i + c * ldg]; acc = fma(g, g, acc); }
+    sig[c] = acc;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double fro2 = 0.0;
+    for (int c = 0; c < n; ++c) fro2 += sig[c];
+    jac_floor_mw = 4.930380657631324e-32 * fro2;   // eps^2 ||G||_F^2
+  }
+  __syncthreads();
+  const double floor2 = jac_floor_mw;
+  const int np = (n + 1) & ~1, npairs = np / 2, m = np - 1;
+  const int pi = tid / TP, tl = tid % TP;
+  const double tol = 2.220446049250313e-16 * sqrt((double)n);
+  int sweeps = -1;
+  for (int s = 0; s < max_sweeps && n > 1; ++s) {
+    int rotated = 0;
+    int pp = pi % m, qq = pi <= m ? (m - pi) % m : 0;
+    for (int r = 0; r < m; ++r) {
+      int p = n, q = n;
+      if (pi < npairs) {
+        if (pi == 0) { p = r; q = m; } else { p = pp; q = qq; }
+        if (p > q) { const int x = p; p = q; q = x; }
+      }
+      pp = pp + 1 == m ? 0 : pp + 1;
+      qq = qq + 1 == m ? 0 : qq + 1;
+      if (q < n) {
+        double gp[UR], gq[UR], jp[HOIST ? UR : 1], jq[HOIST ? UR : 1];
+#pragma unroll
+        for (int u = 0; u < UR; ++u) {
+          const int i = tl + TP * u;
+          gp[u] = i < n ? G[i + p * ldg] : 0.0;
+          gq[u] = i < n ? G[i + q * ldg] : 0.0;
+          if (HOIST) {
+            jp[HOIST ? u : 0] = i < n ? J[i + p * ldj] : 0.0;
+            jq[HOIST ? u : 0] = i < n ? J[i + q * ldj] : 0.0;
+          }
+        }
+        double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+        for (int u = 0; u < UR; ++u) {
+          al = fma(gp[u], gp[u], al); be = fma(gq[u], gq[u], be); ga = fma(gp[u], gq[u], ga);
+        }
+        al = group_sum_dpp<TP>(al); be = group_sum_dpp<TP>(be); ga = group_sum_dpp<TP>(ga);
+        double cs, sn;
+        if (jacobi_rotation(al, be, ga, floor2, tol, &cs, &sn)) {
+#pragma unroll
+          for (int u = 0; u < UR; ++u) {
+            const int i = tl + TP * u;
+            if (i < n) {
+              G[i + p * ldg] = cs * gp[u] - sn * gq[u];
+              G[i + q * ldg] = sn * gp[u] + cs * gq[u];
+              const double a = HOIST ? jp[HOIST ? u : 0] : J[i + p * ldj];
+              const double c = HOIST ? jq[HOIST ? u : 0] : J[i + q * ldj];
+              J[i + p * ldj] = cs * a - sn * c;
+              J[i + q * ldj] = sn * a + cs * c;
+            }
+          }
+          rotated = 1;
+        }
+      }
+      __syncthreads();
+    }
+    // the round's barrier above ordered every G and J access; this one only combines the flags
+    if (!__syncthreads_or(rotated)) { sweeps = s + 1; break; }
+  }
+  if (n <= 1) sweeps = 0;
+  for (int j = tid; j < n; j += nt) {
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) { const double g = G[i + j * ldg]; acc += g * g; }
+    const double sv = sqrt(acc);
+    sig[j] = (sv == sv && sv <= 1.7976931348623157e308) ? sv : -1.0;
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += nt) {
+    const double sj = sig[j];
+    int rank = 0;
+    for (int i = 0; i < n; ++i) {
+      const double si = sig[i];
+      rank += (si > sj) || (si == sj && i < j);
+    }
+    perm[rank] = j;
+  }
+  __syncthreads();   // a full barrier with its memory fence: callers read sig and perm next
+  for (int j = 0; j < n; ++j)
+    if (sig[j] < 0.0) return -2;
+  return sweeps;
+}
+
+// jacobi_svd through its faster forms: one wave (jacobi_svd_w0; 16, 8 or 4 lanes per column pair) when
+// n <= 32, in which case G, J, sig and perm must be in LDS; with 512 threads, 16 lanes per pair up to
+// n = 64 and 8 lanes per pair up to n = 128 (jacobi_svd_mw).  Every thread of the work group calls it; flag
+// is an LDS int (it also carries the one-wave result to the other waves).
 template <int TP>
 __device__ int jacobi_svd_auto(double* G, int ldg, double* J, int ldj, int n, double* sig, int* perm, int* flag,
                                int max_sweeps) {
   if (n <= 32) {
     if (wave_id() == 0) {
-      const int r = n <= 8 ? jacobi_svd_w0<16>(G, ldg, J, ldj, n, sig, perm, max_sweeps)
-                           : (n <= 16 ? jacobi_svd_w0<8>(G, ldg, J, ldj, n, sig, perm, max_sweeps)
-                                      : jacobi_svd_w0<4>(G, ldg, J, ldj, n, sig, perm, max_sweeps));
+      int r;
+      if (n <= 8) r = jacobi_svd_w0<16>(G, ldg, J, ldj, n, sig, perm, max_sweeps);
+      else if (n <= 16) r = jacobi_svd_w0<8>(G, ldg, J, ldj, n, sig, perm, max_sweeps);
+      else r = jacobi_svd_w0<4>(G, ldg, J, ldj, n, sig, perm, max_sweeps);
       if (lane_id() == 0) *flag = r;
     }
     __syncthreads();
     return *flag;
+  }
+  if (blockDim.x >= 512) {
+    if (n <= 64) return jacobi_svd_mw<16, 4>(G, ldg, J, ldj, n, sig, perm, max_sweeps);
+    if (n <= 128) return jacobi_svd_mw<8, 16>(G, ldg, J, ldj, n, sig, perm, max_sweeps);
   }
   return jacobi_svd<TP>(G, ldg, J, ldj, n, sig, perm, flag, max_sweeps);
 }

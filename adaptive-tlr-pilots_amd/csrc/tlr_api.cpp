@@ -81,6 +81,7 @@ struct tlr_ctx {
   int* wlp_count = nullptr;
   int64_t wlp_cap = 0;
   double* scr_pan[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};   // scratch of the panel launches
+  double* scr_help[2] = {nullptr, nullptr};   // scratch of the second 17-32 launches (panel, rest)
   int recomp_grid[RCLASSES] = {0, 0, 0, 0};   // persistent grid of each recompression class
   int* d_retry = nullptr;              // per slot: tiles redone with the wider sketch
   double* d_thr = nullptr;             // per slot (thr_rel, thr_abs) of the current batch
@@ -264,14 +265,18 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
       gsum += c->recomp_grid[q];
     }
     // the classes of one step run concurrently (one stream each) and so do the panel launches, so each
-    // class has its own scratch twice over
-    ACHK(c->alloc(&c->scr_pool, (size_t)2 * gsum * c->scr_stride));
+    // class has its own scratch twice over, and the 17-32 class, launched on two streams, four times
+    ACHK(c->alloc(&c->scr_pool, (size_t)(2 * gsum + 2 * c->recomp_grid[2]) * c->scr_stride));
     int64_t off = 0;
     for (int q = 0; q < RCLASSES; ++q) {
       c->scr_cls[q] = c->scr_pool + off * c->scr_stride;
       off += c->recomp_grid[q];
       c->scr_pan[q] = c->scr_pool + off * c->scr_stride;
       off += c->recomp_grid[q];
+    }
+    for (int h = 0; h < 2; ++h) {
+      c->scr_help[h] = c->scr_pool + off * c->scr_stride;
+      off += c->recomp_grid[2];
     }
     c->cls_stream[0] = c->stream;
     for (int q = 1; q <= 2; ++q) {
@@ -435,17 +440,19 @@ static hipError_t launch_step_lookahead(tlr_ctx* c, const SlotArgs& sa, int k, i
   const bool on = c->tic(5, tt);
   if ((e = launch_recomp_list(sa, k, nslot, c->stream)) != hipSuccess) return e;
   if ((e = hipEventRecord(c->ev_list, c->stream)) != hipSuccess) return e;
-  static const int plan[2][2] = {{2, -1}, {3, 1}};   // classes run by side streams 1 and 2, in order
+  // classes run by side streams 1 and 2, in order; the third entry of stream 2 is a second launch over the
+  // 17-32 lists (claims are atomic, so two launches share a list), which takes what stream 1 has not
+  // started by the time stream 2 is done with its own classes
+  static const int plan[2][3] = {{2, -1, -1}, {3, 1, 2}};
   for (int q = 1; q <= 2; ++q) {
     hipStream_t sq = c->cls_stream[q];
     if ((e = hipStreamWaitEvent(sq, c->ev_list, 0)) != hipSuccess) return e;
     for (int pass = 1; pass >= 0; --pass) {   // panel pairs first, then the rest
-      for (int u = 0; u < 2; ++u) {
+      for (int u = 0; u < 3; ++u) {
         const int cls = plan[q - 1][u];
         if (cls < 0) continue;
-        if ((e = launch_recomp_class(sa, k, cls, pass, c->recomp_grid[cls], pass ? c->scr_pan[cls] : c->scr_cls[cls],
-                                     sq)) != hipSuccess)
-          return e;
+        double* scr = u == 2 ? c->scr_help[pass] : (pass ? c->scr_pan[cls] : c->scr_cls[cls]);
+        if ((e = launch_recomp_class(sa, k, cls, pass, c->recomp_grid[cls], scr, sq)) != hipSuccess) return e;
       }
       if (pass == 1 && (e = hipEventRecord(c->ev_pan[q], sq)) != hipSuccess) return e;
     }
