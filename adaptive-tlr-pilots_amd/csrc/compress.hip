@@ -269,7 +269,10 @@ __global__ __launch_bounds__(256) void k_rank_decide(double* __restrict__ Rm, in
     s_res2 = r2; s_fro2 = f2;
   }
   __syncthreads();
-  if (jacobi_svd<4>(G, k, J, k, k, sig, perm, &flag, 60) == -2) {   // non-finite tile: reported, not stored
+  // k = 64: 32 column pairs of 8 lanes fill the 256 threads (register-held rows); k = 96: the general form
+  const int jr = k <= 64 ? jacobi_svd_mw<8, 8>(G, k, J, k, k, sig, perm, 60)
+                          : jacobi_svd<4>(G, k, J, k, k, sig, perm, &flag, 60);
+  if (jr == -2) {   // non-finite tile: reported, not stored
     if (tid == 0) { atomicMax(&status[ST_GUARD], 2); need[t] = 0; }
     return;
   }
