@@ -23,18 +23,39 @@ __device__ __forceinline__ double matern32(const MaternParams& mp, double x0, do
   return mp.sigma2 * ((1.0 + x) * exp(-x));
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, TLR_WAVE);
+// A double moved between lanes by a DPP pattern within rows of 16 lanes (two 32-bit moves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(x & 0xffffffffll), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over aligned groups of TP (2, 4, 8 or 16) lanes; every lane of a group gets the group's sum.  The
+// exchanges are DPP moves (quad permutations, then mirrors within 8 and 16 lanes), with no LDS round trip.
+template <int TP>
+__device__ __forceinline__ double group_sum_dpp(double v) {
+  static_assert(TP == 2 || TP == 4 || TP == 8 || TP == 16, "group of 2, 4, 8 or 16 lanes");
+  v += dpp_move<0xB1>(v);                    // quad_perm(1, 0, 3, 2)
+  if (TP >= 4) v += dpp_move<0x4E>(v);       // quad_perm(2, 3, 0, 1)
+  if (TP >= 8) v += dpp_move<0x141>(v);      // row_half_mirror: lane l <-> 7 - l within 8
+  if (TP >= 16) v += dpp_move<0x140>(v);     // row_mirror: lane l <-> 15 - l within 16
   return v;
 }
 
-// Sum over groups of G consecutive lanes (G a power of two <= 64).
-template <int G>
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, TLR_WAVE);
-  return v;
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(x & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(x >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes of a wave (all lanes active); every lane gets the sum.  Row sums of 16 lanes by
+// DPP exchanges, then the four row sums read from lanes 0, 16, 32, 48 and added in a fixed order.
+__device__ __forceinline__ double wave_sum(double v) {
+  v = group_sum_dpp<16>(v);
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (TLR_WAVE - 1); }

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(512) void k_potrf(SlotArgs sa, int k) {
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
         if (j < wd) {
-          const double d = __shfl(a[j], j, 64);
+          const double d = readlane_d(a[j], j);   // j is a compile-time lane after unrolling
           if (!(d > 0.0)) fail = 1;
           const double ljj = sqrt(d);
           if (l == j) a[j] = ljj;
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(512) void k_potrf(SlotArgs sa, int k) {
           const double lj = a[j];
 #pragma unroll
           for (int c = j + 1; c < NB; ++c) {
-            const double lcj = __shfl(lj, c, 64);
+            const double lcj = readlane_d(lj, c);
             if (l >= c) a[c] -= lj * lcj;
           }
         }

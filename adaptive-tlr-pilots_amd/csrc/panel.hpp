@@ -269,27 +269,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// A double moved between lanes by a DPP pattern within rows of 16 lanes (two 32-bit moves).
-template <int CTRL>
-__device__ __forceinline__ double dpp_move(double v) {
-  const long long x = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(x & 0xffffffffll), CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, 0xf, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// Sum over aligned groups of TP (2, 4, 8 or 16) lanes; every lane of a group gets the group's sum.  The
-// exchanges are DPP moves (quad permutations, then mirrors within 8 and 16 lanes), with no LDS round trip.
-template <int TP>
-__device__ __forceinline__ double group_sum_dpp(double v) {
-  static_assert(TP == 2 || TP == 4 || TP == 8 || TP == 16, "group of 2, 4, 8 or 16 lanes");
-  v += dpp_move<0xB1>(v);                    // quad_perm(1, 0, 3, 2)
-  if (TP >= 4) v += dpp_move<0x4E>(v);       // quad_perm(2, 3, 0, 1)
-  if (TP >= 8) v += dpp_move<0x141>(v);      // row_half_mirror: lane l <-> 7 - l within 8
-  if (TP >= 16) v += dpp_move<0x140>(v);     // row_mirror: lane l <-> 15 - l within 16
-  return v;
-}
-
 // Rotation of the column pair (p, q) in one-sided Jacobi: given al = ||g_p||^2, be = ||g_q||^2 and
 // ga = g_p . g_q, returns false when the pair is already orthogonal to the tolerance (|ga| <= tol
 // sqrt(al be)) or either column is below the noise floor, else the cosine and sine of the rotation that
