@@ -580,12 +580,13 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
   pan.geqr2(b, r2, vbuf, tau);
   pan.store_r(RuR + r1, w, r2);
   __syncthreads();
-  // ---- 4. core G = R_u R_v^T, one-sided Jacobi, truncation
+  // ---- 4. core K = R_u R_v^T, stored transposed: the one-sided Jacobi runs on K^T (on these cores it
+  // needs fewer sweeps than on K), giving K^T J = V S (columns orthogonal, norms S) and J = U; truncation
   for (int e = tid; e < w * w; e += NT) {
     const int a = e % w, c = e / w;
     double s = (a == c && a < r1) ? sg[a] : 0.0;
     for (int m = 0; m < r2; ++m) s = fma(RuR[a + m * w], RvR[c + m * w], s);
-    G[a + c * w] = s;
+    G[c + a * w] = s;
   }
   __syncthreads();
   const int sweeps = jacobi_svd_auto<L::JTP>(G, w, J, w, w, sig, perm, &flag, 60);
@@ -616,7 +617,8 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
   const int rn = s_rank;
   // ---- 5. outputs, row blocks of RB rows staged through LDS (in place); register row block t of the
   // panel covers rows [64 t, 64 t + 64), so t stays a compile-time index
-  pan.org2r(b, r2, vbuf, tau);   // Q2u in registers
+  // kept column c: U-side coefficients J[:, c] s_c, V-side (K^T J)[:, c] / s_c (orthonormal), scaled as used
+  pan.org2r(b, r2, vbuf, tau);   // Q2u in registers (ends with a work-group barrier)
   const int rbs = 64 * w <= L::FRONT ? 6 : (32 * w <= L::FRONT ? 5 : 4);
   const int RB = 1 << rbs;
 #pragma unroll
@@ -624,7 +626,7 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
     if (64 * t >= b) break;
     const int rend = min(64 * t + 64, b);
     for (int r0 = 64 * t; r0 < rend; r0 += RB) {
-      // V_ij <- [V_ij, Q2v] J[:, perm]
+      // V_ij <- [V_ij, Q2v] V_core[:, perm]
       for (int e = tid; e < RB * w; e += NT) {
         const int rr = e & (RB - 1), m = e >> rbs, p = r0 + rr;
         Sb[m * RB + rr] = (p < b) ? (m < r1 ? Vij[p + (int64_t)m * b] : Q2v[p + (int64_t)(m - r1) * b]) : 0.0;
@@ -632,13 +634,14 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
       __syncthreads();
       for (int e = tid; e < RB * rn; e += NT) {
         const int rr = e & (RB - 1), c = e >> rbs, p = r0 + rr;
-        const double* Jc = J + perm[c] * w;
+        const double* Vc = G + perm[c] * w;
         double s = 0.0;
-        for (int m = 0; m < w; ++m) s = fma(Sb[m * RB + rr], Jc[m], s);
-        if (p < b) Vij[p + (int64_t)c * b] = s;
+        for (int m = 0; m < w; ++m) s = fma(Sb[m * RB + rr], Vc[m], s);
+        const double sv = sig[perm[c]];
+        if (p < b) Vij[p + (int64_t)c * b] = sv > 0.0 ? s / sv : 0.0;
       }
       __syncthreads();
-      // U_ij <- [Q1, Q2u] G[:, perm],  Q1 = U_ij diag(1/sigma)
+      // U_ij <- [Q1, Q2u] (U S)_core[:, perm],  Q1 = U_ij diag(1/sigma)
       for (int e = tid; e < RB * r1; e += NT) {
         const int rr = e & (RB - 1), m = e >> rbs, p = r0 + rr;
         Sb[m * RB + rr] = (p < b) ? Uij[p + (int64_t)m * b] * sc[m] : 0.0;
@@ -652,10 +655,10 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
       __syncthreads();
       for (int e = tid; e < RB * rn; e += NT) {
         const int rr = e & (RB - 1), c = e >> rbs, p = r0 + rr;
-        const double* Gc = G + perm[c] * w;
+        const double* Uc = J + perm[c] * w;
         double s = 0.0;
-        for (int m = 0; m < w; ++m) s = fma(Sb[m * RB + rr], Gc[m], s);
-        if (p < b) Uij[p + (int64_t)c * b] = s;
+        for (int m = 0; m < w; ++m) s = fma(Sb[m * RB + rr], Uc[m], s);
+        if (p < b) Uij[p + (int64_t)c * b] = s * sig[perm[c]];
       }
       __syncthreads();
     }
