@@ -29,10 +29,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "adaptive-tlr-pilots_amd"))
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix), AMD published
-# HBM bytes per recompression step of a batch of 20 theta (FETCH_SIZE + WRITE_SIZE of the step's
-# dispatches averaged over the 78 steps, each counter calibrated against a device copy of known size);
-# measured with rocprofv3 --pmc on this workload, see profiles/recompress_n40000_b500_pmc_traffic.md
-RECOMPRESS_TRAFFIC_PER_STEP = 5.63e9
+# HBM bytes per recompression step and theta (FETCH_SIZE + WRITE_SIZE of the step's dispatches for a
+# batch of 20 theta, averaged over the 78 steps and divided by 20, each counter calibrated against a device
+# copy of known size); measured with rocprofv3 --pmc on this workload, see
+# profiles/recompress_n40000_b500_pmc_traffic.md.  A launch group covers every theta of one batch (20 for
+# log L; with the gradient, 20 in the base batch and 80 in the replay batch, the same ranks per theta).
+RECOMPRESS_TRAFFIC_PER_STEP_THETA = 5.63e9 / 20
 
 
 def lhs_thetas(m, seed=0):
@@ -143,6 +145,9 @@ def main():
     il = np.tril_indices(nb, -1)
     mean_rank = float(r["ranks_factor"][:, il[0], il[1]].mean())
     achieved = flops / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
+    # theta-steps per timed launch group (steps with recompressions: k = 0 .. nB - 3)
+    theta_steps = a.batch * (5 if grad else 1) * a.steps * max(nb - 2, 0)
+    traffic = RECOMPRESS_TRAFFIC_PER_STEP_THETA * theta_steps / max(launches, 1)
     p_achieved = pflops / (pms * 1e-3) / 1e12 if pms > 0 else 0.0
     out = {
         "metric": "TLR logL+grad evals/s at N=40k, mixed vs plain FP64 GPU, |dlogL|<=1e-2",
@@ -158,7 +163,7 @@ def main():
                                       "and |dlogL| up to 4.8e-2 at this configuration",
                    "completed": int(r["completed"].sum()), "mean_rank_factor": mean_rank},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": RECOMPRESS_TRAFFIC_PER_STEP,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "k_recompress (work lists + 4 width classes, one launch group per step)",
                      "launches": launches, "kernel_ms": kms,
                      "flops_per_launch": flops / max(launches, 1)},
