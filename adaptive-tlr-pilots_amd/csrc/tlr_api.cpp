@@ -1,10 +1,11 @@
 // C ABI of the TLR likelihood: context management and the per-evaluation launch schedule.
 //
-// Schedule of one batch of m theta (one slot per theta), all on the context's stream:
+// Schedule of one batch of m theta (one slot per theta), on the context's stream (and two side streams):
 //   per slot: diagonal tiles; initial compression of every off-diagonal tile (sketch width 64, then 96
 //             for the tiles whose rank left too little oversampling)
-//   for k = 0 .. nB-1 (all slots in each launch): POTRF(k) -> TRSM + forward solve(k) ->
-//             diagonal updates(k) -> recompressions(k) (three width classes)
+//   for k = 0 .. nB-1 (all slots in each launch): POTRF(k) -> inverse of L_kk -> TRSM + forward solve(k) ->
+//             diagonal updates(k) -> recompressions(k) (four width classes on three streams, with the next
+//             step's POTRF, inverse, TRSM and diagonal update overlapped)
 //   final reductions; one synchronisation and a small copy back.
 #include <hip/hip_runtime.h>
 #include <math.h>
