@@ -441,7 +441,7 @@ __global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
 // (block Gram-Schmidt applied twice, Householder QR of the r_jk new columns only).  The core
 // R_u R_v^T (w x w, w = r_ij + r_jk) goes to the one-sided Jacobi SVD; the outputs are formed row block
 // by row block through LDS, in place.  The panel holds the r_jk new columns in NW waves x CPW column
-// slots: r_jk <= 8 (nine pairs in ten) in 4 waves x 2 slots with a 52 KB LDS footprint, so that three
+// slots: r_jk <= 8 (nine pairs in ten) in 4 waves x 2 slots with a 36 KB LDS footprint, so that four
 // work groups share a CU; r_jk <= 16 in 4 x 4 (two per CU); r_jk <= 32 in 8 x 4 and r_jk <= 64 in 8 x 8
 // (one per CU, cores up to 64 x 64 in LDS).  The pairs of each class come from a work list.
 template <int NW, int CPW>
@@ -450,12 +450,13 @@ struct RecompLayout {
   static constexpr int WMAX = 64 + R2MAX;
   // cores up to WJ x WJ run in LDS (larger ones in the work group's scratch)
   // The 8-wave classes hold a CU's whole register file, so nothing else shares their CU: they take the
-  // LDS as well (cores up to 64 x 64); the 4-wave classes keep to 52-63 KB (cores up to 32 x 32).
+  // LDS as well (cores up to 64 x 64); the 4-wave classes keep to 36-56 KB (cores up to 32 x 32).
   static constexpr int WJ = NW >= 8 ? 64 : 32;
   static constexpr bool RR_LDS = R2MAX <= 32;
-  // front: W and the CGS coefficients (2 x 64 x R2MAX), later the output staging (RB rows x w)
   static constexpr bool WC_LDS = true;
-  static constexpr int FRONT = (WC_LDS && 2 * 64 * R2MAX > 2048) ? 2 * 64 * R2MAX : 2048;
+  // front: W and the CGS coefficients (rik x r2 and r1 x r2, 64 x R2MAX each), later the output staging
+  // (RB x w, RB = 64 .. 8 rows as w allows)
+  static constexpr int FRONT = 2 * 64 * R2MAX > 1024 ? 2 * 64 * R2MAX : 1024;
   static constexpr int RR = RR_LDS ? WMAX * R2MAX : 0;
   static constexpr int OFF_RV = FRONT, OFF_RU = OFF_RV + RR, OFF_SC = OFF_RU + RR, OFF_SG = OFF_SC + 64,
                        OFF_G = OFF_SG + 64, OFF_J = OFF_G + WJ * WJ, TOTAL = OFF_J + WJ * WJ;
@@ -524,7 +525,9 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
   const bool core_lds = w <= L::WJ;
   double* G = core_lds ? lds + L::OFF_G : scr + (int64_t)b * 64 + 2 * 128 * 64;
   double* J = core_lds ? lds + L::OFF_J : G + 128 * 128;
-  __shared__ double vbuf[2 * 64 * RPL];
+  // Householder vectors of the panel QRs (phases 2-3 only): the core's LDS region, unused until phase 4
+  static_assert(2 * 64 * RPL <= L::WJ * L::WJ, "reflector buffer must fit the core region");
+  double* vbuf = lds + L::OFF_G;
   __shared__ double tau[8 * NW];
   __shared__ double sig[L::WMAX];
   __shared__ int perm[L::WMAX];
@@ -579,7 +582,7 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
   }
   pan.geqr2(b, r2, vbuf, tau);
   pan.store_r(RuR + r1, w, r2);
-  __syncthreads();
+  pan.org2r(b, r2, vbuf, tau);   // Q2u in registers from here to the outputs (ends with a barrier)
   // ---- 4. core K = R_u R_v^T, stored transposed: the one-sided Jacobi runs on K^T (on these cores it
   // needs fewer sweeps than on K), giving K^T J = V S (columns orthogonal, norms S) and J = U; truncation
   for (int e = tid; e < w * w; e += NT) {
@@ -618,8 +621,7 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
   // ---- 5. outputs, row blocks of RB rows staged through LDS (in place); register row block t of the
   // panel covers rows [64 t, 64 t + 64), so t stays a compile-time index
   // kept column c: U-side coefficients J[:, c] s_c, V-side (K^T J)[:, c] / s_c (orthonormal), scaled as used
-  pan.org2r(b, r2, vbuf, tau);   // Q2u in registers (ends with a work-group barrier)
-  const int rbs = 64 * w <= L::FRONT ? 6 : (32 * w <= L::FRONT ? 5 : 4);
+  const int rbs = 64 * w <= L::FRONT ? 6 : (32 * w <= L::FRONT ? 5 : (16 * w <= L::FRONT ? 4 : 3));
   const int RB = 1 << rbs;
 #pragma unroll
   for (int t = 0; t < RPL; ++t) {
@@ -673,11 +675,11 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
 }
 
 // Persistent grid over the class-cls work list of step k (grid = resident work groups of the class).
-// The narrow classes ask for 3 (r_jk <= 8) and 2 (r_jk <= 16) waves per SIMD so that the register
+// The narrow classes ask for 4 (r_jk <= 8) and 2 (r_jk <= 16) waves per SIMD so that the register
 // allocation leaves room for as many work groups as their LDS footprint allows.
 template <int NW, int CPW>
 struct RecompWaves {
-  static constexpr int value = NW * CPW <= 8 ? 3 : (NW * CPW <= 16 ? 2 : 1);
+  static constexpr int value = NW * CPW <= 8 ? 4 : (NW * CPW <= 16 ? 2 : 1);
 };
 
 // Entries are claimed one at a time from the list's claim counter, in list order (heavy pairs first), so a
