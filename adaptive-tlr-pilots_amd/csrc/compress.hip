@@ -44,16 +44,23 @@ __global__ void k_gen_diag(const double* __restrict__ sx, const double* __restri
 // --------------------------------------------------------------------------------- Matern products
 // out_t (b x k) = op(T_t) X_t for every active tile t.  op = 0: T (rows: sites of tile i, inner index:
 // sites of tile j); op = 1: T^T (the roles of the two site sets swap).  X_t has leading dimension b;
-// xstride = 0 shares one X (the Gaussian test matrix) across tiles.  Work group: 256 threads computing a
-// 256-row x 64-column block as 8 x 8 register blocks (rows strided by 32 so that stores coalesce); the
-// inner index runs in chunks of 16 whose kernel entries are generated once into LDS.
-constexpr int MP_ROWS = 256, MP_COLS = 64, MP_QC = 16;
+// xstride = 0 shares one X (the Gaussian test matrix) across tiles.
+//
+// On the matrix cores (v_mfma_f64_16x16x4f64), computing out^T = X^T op(T)^T so that the accumulators hold
+// 16 consecutive rows of one output column per 16 lanes (stores in 128-byte runs).  Work group: 4 waves,
+// each 32 rows x 64 columns (2 x 4 accumulator blocks of 16 x 16); the work group covers 128 rows.  Per step
+// of 4 inner indices a lane generates the 2 kernel entries of its operand (rows p0 + lane % 16 and
+// p0 + 16 + lane % 16, inner index lane / 16) and reads 4 operands of X from LDS; the 8 MFMAs reuse each
+// generated entry for 64 columns, and the entry generation (VALU) overlaps the matrix pipe.  On 3160-tile
+// tests its outputs equal those of a sequential FMA loop over the inner index bit for bit.
+constexpr int MM_ROWS = 128, MM_COLS = 64, MM_QC = 64, MM_LD = MM_COLS + 1;
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void k_matern_product(const double* __restrict__ sx, const double* __restrict__ sy,
-                                                        int64_t n, int b, MaternParams mp, int op,
-                                                        const double* __restrict__ X, int64_t xstride,
-                                                        double* __restrict__ out, int64_t ostride, int k,
-                                                        const int* __restrict__ active) {
+__global__ __launch_bounds__(256, 4) void k_matern_product(const double* __restrict__ sx, const double* __restrict__ sy,
+                                                           int64_t n, int b, MaternParams mp, int op,
+                                                           const double* __restrict__ X, int64_t xstride,
+                                                           double* __restrict__ out, int64_t ostride, int k,
+                                                           const int* __restrict__ active) {
   const int64_t t = blockIdx.z;
   if (active && !active[t]) return;
   int i, j;
@@ -61,75 +68,64 @@ __global__ __launch_bounds__(256) void k_matern_product(const double* __restrict
   if (op) { const int x = i; i = j; j = x; }
   const int64_t r0g = (int64_t)i * b, q0g = (int64_t)j * b;
   const int mr = (int)min<int64_t>(b, n - r0g), mq = (int)min<int64_t>(b, n - q0g);
-  const int rb = blockIdx.x * MP_ROWS, cb = blockIdx.y * MP_COLS;
-  __shared__ double Ts[MP_QC][MP_ROWS];
-  __shared__ double Xs[MP_QC][MP_COLS];
-  __shared__ double rx[MP_ROWS], ry[MP_ROWS];
-  __shared__ double qx[MP_QC], qy[MP_QC];
-  const int tid = threadIdx.x;
-  // thread (tr, tc) owns rows rb + 32 a + tr (a < 8) and columns cb + 8 tc + c (c < 8): the 32 lanes of a
-  // column group read consecutive LDS words and store 32 consecutive rows of one column (256 bytes)
-  const int tr = tid % 32, tc = tid / 32;
-  {
-    const int p = rb + tid;
-    rx[tid] = (p < mr) ? sx[r0g + p] : 0.0;
-    ry[tid] = (p < mr) ? sy[r0g + p] : 0.0;
-  }
+  const int cb = blockIdx.y * MM_COLS;
+  const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63, lr = ln & 15, lq = ln >> 4;
+  const int p0 = blockIdx.x * MM_ROWS + 32 * wv;   // this wave's first row
+  __shared__ double Xs[MM_QC * MM_LD];             // X chunk, [inner][column]
+  __shared__ double qxs[MM_QC], qys[MM_QC];
+  // the lane's two rows (sites); rows past the tile generate zeros
+  const int pa = p0 + lr, pb = p0 + 16 + lr;
+  const bool oka = pa < mr, okb = pb < mr;
+  const double ax = oka ? sx[r0g + pa] : 0.0, ay = oka ? sy[r0g + pa] : 0.0;
+  const double bx = okb ? sx[r0g + pb] : 0.0, by = okb ? sy[r0g + pb] : 0.0;
   const double* Xt = X + t * xstride;
-  double acc[8][8];
+  dbl4 acc[2][4];
 #pragma unroll
-  for (int a = 0; a < 8; ++a)
+  for (int f = 0; f < 2; ++f)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) acc[a][c] = 0.0;
-  __syncthreads();
-  for (int q0 = 0; q0 < mq; q0 += MP_QC) {
-    if (tid < MP_QC) {
+    for (int c = 0; c < 4; ++c) acc[f][c] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  for (int q0 = 0; q0 < mq; q0 += MM_QC) {
+    __syncthreads();   // the previous chunk's reads are done
+    for (int e = tid; e < MM_QC * MM_COLS; e += 256) {
+      const int qq = e % MM_QC, cc = e / MM_QC;
+      const int q = q0 + qq, c = cb + cc;
+      Xs[qq * MM_LD + cc] = (q < mq && c < k) ? Xt[q + (int64_t)c * b] : 0.0;
+    }
+    if (tid < MM_QC) {
       const int q = q0 + tid;
-      qx[tid] = (q < mq) ? sx[q0g + q] : 0.0;
-      qy[tid] = (q < mq) ? sy[q0g + q] : 0.0;
+      qxs[tid] = (q < mq) ? sx[q0g + q] : 0.0;
+      qys[tid] = (q < mq) ? sy[q0g + q] : 0.0;
     }
     __syncthreads();
-    // kernel entries: 256 rows x 16 inner indices; thread tid generates row tid
-    {
-      const bool rowok = rb + tid < mr;
-      const double px = rx[tid], py = ry[tid];
-#pragma unroll 4
-      for (int qq = 0; qq < MP_QC; ++qq) {
-        double v = 0.0;
-        if (rowok && q0 + qq < mq) v = matern32(mp, px, py, qx[qq], qy[qq]);
-        Ts[qq][tid] = v;
+    const int qn = min(MM_QC, mq - q0);
+    for (int s = 0; s < qn; s += 4) {
+      const int qq = s + lq;
+      const bool okq = qq < qn;
+      const double qx = qxs[qq], qy = qys[qq];
+      const double ta = (oka && okq) ? matern32(mp, ax, ay, qx, qy) : 0.0;
+      const double tb = (okb && okq) ? matern32(mp, bx, by, qx, qy) : 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double xv = Xs[qq * MM_LD + 16 * c + lr];   // X^T[col 16 c + lane % 16][inner lane / 16]
+        acc[0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv, ta, acc[0][c], 0, 0, 0);
+        acc[1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv, tb, acc[1][c], 0, 0, 0);
       }
     }
-    for (int e = tid; e < MP_QC * MP_COLS; e += 256) {
-      const int qq = e % MP_QC, cc = e / MP_QC;
-      const int q = q0 + qq, c = cb + cc;
-      Xs[qq][cc] = (q < mq && c < k) ? Xt[q + (int64_t)c * b] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int qq = 0; qq < MP_QC; ++qq) {
-      double tv[8], xv[8];
-#pragma unroll
-      for (int a = 0; a < 8; ++a) tv[a] = Ts[qq][32 * a + tr];
-#pragma unroll
-      for (int c = 0; c < 8; ++c) xv[c] = Xs[qq][tc * 8 + c];
-#pragma unroll
-      for (int a = 0; a < 8; ++a)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) acc[a][c] = fma(tv[a], xv[c], acc[a][c]);
-    }
-    __syncthreads();
   }
+  // accumulator (f, c), item v: column cb + 16 c + lane / 16 + 4 v, row p0 + 16 f + lane % 16 (the MFMA's
+  // result layout: item v of lane l holds D[l / 16 + 4 v][l % 16])
   double* Ot = out + t * ostride;
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    const int p = rb + 32 * a + tr;
+  for (int f = 0; f < 2; ++f) {
+    const int p = p0 + 16 * f + lr;
     if (p < b) {
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int cc = cb + tc * 8 + c;
-        if (cc < k) Ot[p + (int64_t)cc * b] = (p < mr) ? acc[a][c] : 0.0;
-      }
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int col = cb + 16 * c + lq + 4 * v;
+          if (col < k) Ot[p + (int64_t)col * b] = (p < mr) ? acc[f][c][v] : 0.0;
+        }
     }
   }
 }
@@ -260,12 +256,15 @@ __global__ __launch_bounds__(256) void k_rank_decide(double* __restrict__ Rm, in
   const double* Rt = Rm + t * rstride;
   for (int e = tid; e < k * k; e += blockDim.x) G[e] = Rt[e];
   if (tid == 0) {
+    // residual and Frobenius norm from k_residual's partial sums (a forced rank needs neither, and the
+    // residual pass is not run for it)
     const int nbb = (b + 63) / 64;
     double r2 = 0.0, f2 = 0.0;
-    for (int e = 0; e < nbb * nbb; ++e) {
-      r2 += part[(t * nbb * nbb + e) * 2 + 0];
-      f2 += part[(t * nbb * nbb + e) * 2 + 1];
-    }
+    if (!forced)
+      for (int e = 0; e < nbb * nbb; ++e) {
+        r2 += part[(t * nbb * nbb + e) * 2 + 0];
+        f2 += part[(t * nbb * nbb + e) * 2 + 1];
+      }
     s_res2 = r2; s_fro2 = f2;
   }
   __syncthreads();
@@ -401,7 +400,7 @@ hipError_t launch_gen_diag(const double* sx, const double* sy, int64_t n, int b,
 hipError_t launch_matern_product(const double* sx, const double* sy, int64_t n, int b, int64_t ntile, MaternParams mp,
                                  int op, const double* X, int64_t xstride, double* out, int64_t ostride, int k,
                                  const int* active, hipStream_t s) {
-  dim3 grid(nblk(b, MP_ROWS), nblk(k, MP_COLS), (unsigned)ntile);
+  dim3 grid(nblk(b, MM_ROWS), nblk(k, MM_COLS), (unsigned)ntile);
   hipLaunchKernelGGL(k_matern_product, grid, dim3(256), 0, s, sx, sy, n, b, mp, op, X, xstride, out, ostride, k,
                      active);
   return hipGetLastError();
