@@ -17,6 +17,7 @@
 namespace tlr {
 
 constexpr int NB = 32;   // inner panel width of the diagonal-tile factorization
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 // status[ST_FAILED] holds the first failing diagonal tile (INT_MAX when none).
 __device__ __forceinline__ bool failed(const int* status) {
@@ -127,45 +128,62 @@ __global__ __launch_bounds__(512) void k_potrf(SlotArgs sa, int k) {
       }
     }
     __syncthreads();
-    // trailing update of the lower triangle of rows/cols >= p0 + wd, 8 x 8 blocks
+    // trailing update A_22 -= L_21 L_21^T of the lower triangle of rows/cols >= p0 + wd on the matrix cores:
+    // 16 x 16 blocks (I >= J), four per wave at a time (their A entries are loaded before the products so
+    // that the loads' latency overlaps the MFMA chains).  The MFMA (v_mfma_f64_16x16x4f64) forms
+    // D[x][y] = sum_c L[t0 + 16 J + x][c] L[t0 + 16 I + y][c], 4 panel columns per instruction, so item v of
+    // lane l is entry (t0 + 16 I + l % 16, t0 + 16 J + l / 16 + 4 v): 16 consecutive rows of a column per 16
+    // lanes.  The chained products add in the panel-column order of a sequential FMA loop.
     const int t0 = p0 + wd;
     const int m = b - t0;
     if (m > 0) {
-      const int n8 = (m + 7) / 8;
-      const int nblk8 = n8 * (n8 + 1) / 2;
-      for (int e = tid; e < nblk8; e += blockDim.x) {
-        int I = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-        while ((I + 1) * (I + 2) / 2 <= e) ++I;
-        while (I * (I + 1) / 2 > e) --I;
-        const int J = e - I * (I + 1) / 2;
-        const int r0 = t0 + 8 * I, c0 = t0 + 8 * J;
-        double acc[8][8];
+      constexpr int GB = 4;
+      const int n16 = (m + 15) / 16;
+      const int nblk16 = n16 * (n16 + 1) / 2;
+      const int lr = l & 15, lq = l >> 4;
+      for (int e0 = GB * w; e0 < nblk16; e0 += GB * 8) {
+        int ri[GB], rj[GB], cj[GB];
+        double cv[GB][4];
 #pragma unroll
-        for (int a = 0; a < 8; ++a)
+        for (int g = 0; g < GB; ++g) {
+          const int e = e0 + g;
+          ri[g] = b;   // blocks past the end: every entry masked
+          rj[g] = b;
+          cj[g] = b;
+          if (e < nblk16) {
+            int I = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+            while ((I + 1) * (I + 2) / 2 <= e) ++I;
+            while (I * (I + 1) / 2 > e) --I;
+            const int J = e - I * (I + 1) / 2;
+            ri[g] = t0 + 16 * I + lr;
+            rj[g] = t0 + 16 * J + lr;
+            cj[g] = t0 + 16 * J + lq;
+          }
 #pragma unroll
-          for (int c = 0; c < 8; ++c) acc[a][c] = 0.0;
-        for (int cc = 0; cc < wd; ++cc) {
-          double li[8], lj[8];
-#pragma unroll
-          for (int a = 0; a < 8; ++a) li[a] = (r0 + a < b) ? Lp[cc][r0 + a] : 0.0;
-#pragma unroll
-          for (int c = 0; c < 8; ++c) lj[c] = (c0 + c < b) ? Lp[cc][c0 + c] : 0.0;
-#pragma unroll
-          for (int a = 0; a < 8; ++a)
-#pragma unroll
-            for (int c = 0; c < 8; ++c) acc[a][c] = fma(li[a], lj[c], acc[a][c]);
-        }
-#pragma unroll
-        for (int a = 0; a < 8; ++a) {
-          const int r = r0 + a;
-          if (r < b) {
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-              const int q = c0 + c;
-              if (q <= r) A[r + (int64_t)q * b] -= acc[a][c];
-            }
+          for (int v = 0; v < 4; ++v) {
+            const int jc = cj[g] + 4 * v;
+            cv[g][v] = (ri[g] < b && jc <= ri[g]) ? A[ri[g] + (int64_t)jc * b] : 0.0;
           }
         }
+        dbl4 acc[GB];
+#pragma unroll
+        for (int g = 0; g < GB; ++g) acc[g] = (dbl4){0.0, 0.0, 0.0, 0.0};
+        for (int s4 = 0; s4 < wd; s4 += 4) {
+          const int c = s4 + lq;
+#pragma unroll
+          for (int g = 0; g < GB; ++g) {
+            const double av = (rj[g] < b && c < wd) ? Lp[c][rj[g]] : 0.0;
+            const double bv = (ri[g] < b && c < wd) ? Lp[c][ri[g]] : 0.0;
+            acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[g], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < GB; ++g)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int jc = cj[g] + 4 * v;
+            if (ri[g] < b && jc <= ri[g]) A[ri[g] + (int64_t)jc * b] = cv[g][v] - acc[g][v];
+          }
       }
     }
     __syncthreads();
