@@ -133,15 +133,15 @@ __global__ __launch_bounds__(256, 4) void k_matern_product(const double* __restr
 // ------------------------------------------------------------------------------ orthonormalisation
 // In place: M_t (b x k) -> Q_t (explicit, b x k) of its Householder QR; optionally R_t (k x k, col-major)
 // stored to R (transposed when r_transposed).
-template <int RPL, int NW>
+template <int RPL, int NW, int CPW>
 __global__ __launch_bounds__(64 * NW) void k_panel_qr(double* __restrict__ M, int64_t mstride, int b, int k,
                                                         double* __restrict__ R, int64_t rstride, int r_transposed,
                                                         const int* __restrict__ active) {
   const int64_t t = blockIdx.x;
   if (active && !active[t]) return;
   __shared__ double vbuf[2 * 64 * RPL];
-  __shared__ double tau[8 * NW];
-  RegPanel<RPL, NW> P;
+  __shared__ double tau[NW * CPW];
+  RegPanel<RPL, NW, CPW> P;
   double* Mt = M + t * mstride;
   P.load(Mt, b, b, k);
   P.geqr2(b, k, vbuf, tau);
@@ -406,11 +406,11 @@ hipError_t launch_matern_product(const double* sx, const double* sy, int64_t n, 
   return hipGetLastError();
 }
 
-template <int RPL, int NW>
+template <int RPL, int NW, int CPW = 8>
 static void launch_qr_t(double* M, int64_t mstride, int b, int k, double* R, int64_t rstride, int rt,
                         const int* active, int64_t ntile, hipStream_t s) {
-  hipLaunchKernelGGL((k_panel_qr<RPL, NW>), dim3((unsigned)ntile), dim3(64 * NW), 0, s, M, mstride, b, k, R, rstride,
-                     rt, active);
+  hipLaunchKernelGGL((k_panel_qr<RPL, NW, CPW>), dim3((unsigned)ntile), dim3(64 * NW), 0, s, M, mstride, b, k, R,
+                     rstride, rt, active);
 }
 
 hipError_t launch_panel_qr(double* M, int64_t mstride, int b, int k, double* R, int64_t rstride, int r_transposed,
@@ -423,8 +423,10 @@ hipError_t launch_panel_qr(double* M, int64_t mstride, int b, int k, double* R, 
     else if (nw <= 12) launch_qr_t<4, 12>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
     else launch_qr_t<4, 16>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
   } else {
+    // 64 columns: 16 waves of 4 columns (a quarter of the register panel per wave of the 8 x 8 form, so that
+    // four waves share each SIMD and the per-column reflector work is split over twice as many waves)
     if (nw <= 4) launch_qr_t<8, 4>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
-    else if (nw <= 8) launch_qr_t<8, 8>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
+    else if (nw <= 8) launch_qr_t<8, 16, 4>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
     else if (nw <= 12) launch_qr_t<8, 12>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
     else launch_qr_t<8, 16>(M, mstride, b, k, R, rstride, r_transposed, active, ntile, s);
   }
