@@ -367,6 +367,25 @@ __global__ void k_stash(double* __restrict__ U, double* __restrict__ V, int64_t 
   if (dir == 1 && blockIdx.x == 0 && threadIdx.x == 0) { rank[t] = r; rank0[t] = r; }
 }
 
+// Warm-started sketch of a replay with given ranks: X_t = [V_t of the recorded evaluation (its r0_t stored
+// columns), the test matrix's columns r0_t .. k-1], so that the first product T X_t already spans the
+// dominant range of T up to the change in theta (the right factor V_t spans the recorded tile's row space).
+__global__ void k_warm_sketch(const double* __restrict__ stash, const int64_t* __restrict__ off,
+                             const int* __restrict__ r0, const double* __restrict__ omega, double* __restrict__ X,
+                             int64_t xstride, int b, int k, const int* __restrict__ active) {
+  const int64_t t = blockIdx.y;
+  if (active && !active[t]) return;
+  const int r = min(r0[t], k);
+  const int64_t cnt = (int64_t)b * r0[t];
+  const double* Vs = stash + 2 * (int64_t)b * off[t] + cnt;   // V part of the record's tile t (b x r0)
+  double* Xt = X + t * xstride;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)b * k;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e / b);
+    Xt[e] = c < r ? Vs[e] : omega[e];
+  }
+}
+
 // ------------------------------------------------------------------------------------ launchers
 static int nblk(int64_t a, int64_t d) { return (int)((a + d - 1) / d); }
 
@@ -375,6 +394,14 @@ hipError_t launch_stash(double* U, double* V, int64_t fstride, int b, const int*
   if (ntile <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_stash, dim3(4, (unsigned)ntile), dim3(256), 0, s, U, V, fstride, b, r0, off, stash, rank, rank0,
                      dir);
+  return hipGetLastError();
+}
+
+hipError_t launch_warm_sketch(const double* stash, const int64_t* off, const int* r0, const double* omega, double* X,
+                              int64_t xstride, int b, int k, const int* active, int64_t ntile, hipStream_t s) {
+  if (ntile <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_warm_sketch, dim3(nblk((int64_t)b * k, 256 * 8), (unsigned)ntile), dim3(256), 0, s, stash, off,
+                     r0, omega, X, xstride, b, k, active);
   return hipGetLastError();
 }
 

@@ -362,8 +362,11 @@ int tlr_ctx_set_y(tlr_ctx* ctx, const double* y) {
   return TLR_OK;
 }
 
+// warm >= 0 (a replay whose ranks are given, forced != nullptr): the sketch starts from the right factors of
+// stash record `warm` (the recorded evaluation at a nearby theta), and two products (T X, T^T Q) replace the
+// four of a cold start.
 static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternParams& mp, int rule_fixed,
-                         double thr_rel, double thr_abs, const int* forced, int mixed) {
+                         double thr_rel, double thr_abs, const int* forced, int mixed, int warm = -1) {
   hipStream_t s = c->stream;
   const int b = c->b;
   const int64_t nt = c->ntile;
@@ -382,11 +385,19 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
     const double gemm_flops = 2.0 * b * (double)b * k;     // one product with a b x b tile
     const double qr_flops = 4.0 * b * (double)k * k;        // Householder QR + explicit Q
     TimedLaunch tl{nullptr, nullptr, slot, rd, gemm_flops};
+    const bool warm_start = warm >= 0 && forced;
+    if (warm_start)
+      HIPCHK(launch_warm_sketch(c->stash, c->d_stash_off + (int64_t)warm * nt, c->rep_rank0 + (int64_t)warm * nt,
+                                c->omega, oth, ms, b, k, c->need, nt, s));
     bool on = c->tic(1, tl);
-    HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, 0, c->omega, 0, cur, ms, k, c->need, s));
+    if (warm_start)
+      HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, 0, oth, ms, cur, ms, k, c->need, s));
+    else
+      HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, 0, c->omega, 0, cur, ms, k, c->need, s));
     c->toc(on, tl);
     int last_op = 0;
-    for (int pidx = 1; pidx < NPRODUCT; ++pidx) {
+    const int nproduct = warm_start ? 2 : NPRODUCT;
+    for (int pidx = 1; pidx < nproduct; ++pidx) {
       double* src = (last_op == 0) ? cur : oth;
       double* dst = (last_op == 0) ? oth : cur;
       TimedLaunch tq{nullptr, nullptr, slot, rd, qr_flops};
@@ -540,7 +551,7 @@ static int stash_compressed(tlr_ctx* c, int m) {
 // stash != 0 stores every slot's compressed factors as records 0..m-1.
 static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
                      const int* replay, double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma,
-                     int32_t* ranks_factor, const int* reuse = nullptr, int stash = 0) {
+                     int32_t* ranks_factor, const int* reuse = nullptr, int stash = 0, const int* warm = nullptr) {
   if (!c || !thetas || !out || m < 1 || m > c->max_batch) return set_err(TLR_ERR_ARG, "invalid argument");
   if (rule != TLR_RULE_FIXED && rule != TLR_RULE_ADAPTIVE && rule != TLR_RULE_MIXED)
     return set_err(TLR_ERR_ARG, "unknown rule");
@@ -590,7 +601,7 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
       HIPCHK(launch_stash(p.U, p.V, (int64_t)c->b * RANK_CAP, c->b, c->rep_rank0 + (int64_t)rec * c->ntile,
                           c->d_stash_off + (int64_t)rec * c->ntile, c->stash, p.rank, p.rank0, c->ntile, 1, st));
     } else {
-      int rc = compress_slot(c, s, p, mp, rule_fixed, value, thr_abs, forced, mixed);
+      int rc = compress_slot(c, s, p, mp, rule_fixed, value, thr_abs, forced, mixed, warm ? warm[s] : -1);
       if (rc != TLR_OK) return rc;
     }
   }
@@ -740,7 +751,7 @@ int tlr_loglik_grad_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
   // d/dlog range, d/dlog eta: central differences at the base ranks; the eta perturbations leave the
   // off-diagonal tiles unchanged, so they reuse the base's compressed factors
   std::vector<double> th4;
-  std::vector<int> src, reuse4;
+  std::vector<int> src, reuse4, warm4;
   for (int s = 0; s < m; ++s) {
     grad[3 * s + 0] = st[s] == TLR_OK ? 0.5 * out[3 * s + 2] - 0.5 * (double)c->n : NAN;
     grad[3 * s + 1] = grad[3 * s + 2] = NAN;
@@ -751,7 +762,8 @@ int tlr_loglik_grad_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
         th4.insert(th4.end(), {t[0], t[1], t[2]});
         th4[th4.size() - 3 + par] *= exp(sg * h);
         src.push_back(s);
-        reuse4.push_back(par == 2 ? s : -1);
+        reuse4.push_back(par == 2 ? s : -1);   // eta: the base's compressed factors as they are
+        warm4.push_back(par == 1 ? s : -1);    // range: sketch warm-started from the base's factors
       }
   }
   const int n4 = (int)src.size();
@@ -760,7 +772,7 @@ int tlr_loglik_grad_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
   for (int off = 0; off < n4; off += c->max_batch) {
     const int mm = n4 - off < c->max_batch ? n4 - off : c->max_batch;
     rc = run_batch(c, mm, th4.data() + 3 * off, rule, value, precision, src.data() + off, o4.data() + 3 * off,
-                   s4.data() + off, f4.data() + off, nullptr, nullptr, reuse4.data() + off, 0);
+                   s4.data() + off, f4.data() + off, nullptr, nullptr, reuse4.data() + off, 0, warm4.data() + off);
     if (rc < 0) return rc;
   }
   for (int e = 0; e < n4; e += 4) {

@@ -81,6 +81,8 @@ hipError_t launch_rank_decide(double* Rm, int64_t rstride, const double* Qrow, c
                               double thr_rel, double thr_abs, double* U, double* V, int64_t fstride, int rank_cap,
                               int* rank, int* need, int* retry_count, int* status, const int* forced, int mixed,
                               int64_t ntile, hipStream_t s);
+hipError_t launch_warm_sketch(const double* stash, const int64_t* off, const int* r0, const double* omega, double* X,
+                              int64_t xstride, int b, int k, const int* active, int64_t ntile, hipStream_t s);
 hipError_t launch_fill_int(int* p, int64_t count, int value, hipStream_t s);
 hipError_t launch_stash(double* U, double* V, int64_t fstride, int b, const int* r0, const int64_t* off,
                         double* stash, int* rank, int* rank0, int64_t ntile, int dir, hipStream_t s);
