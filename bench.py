@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the adaptive-rank TLR log-likelihood on one MI355X.
 
-Workload (the BASELINE configuration of the plain TLR Cholesky with recompression): N = 40,000 sites
+Workload (the plain FP64 TLR Cholesky with recompression at N = 40,000, b = 500): N = 40,000 sites
 i.i.d. uniform in the unit square (default_rng(0)), Morton-ordered, Matern nu=3/2, tiles b = 500
 (80 x 80 tile grid), adaptive rank rule tau = 0.9, ranks re-chosen at each of 20 theta on a Latin
 hypercube (scipy.stats.qmc.LatinHypercube(d=3, seed=0)) over sigma2 in [0.5, 2], range in [0.1, 0.6],
@@ -107,8 +107,8 @@ def main():
     ap.add_argument("--no-grad", action="store_true", help="time log L alone")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: multi-GPU runs are not part of this build (see the sharding notes "
-              "in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: multi-GPU runs are not supported (one GPU's share of the path is "
+              "what this library runs)", file=sys.stderr)
         sys.exit(2)
     import tlrgp
     S, y = load_field(a.n)

@@ -1,4 +1,4 @@
-"""Oracle results on the seed-0 N=40,000 field at the north-star tile size b=500 (adaptive tau=0.9,
+"""Oracle results on the seed-0 N=40,000 field at the benchmark tile size b=500 (adaptive tau=0.9,
 theta=(1, 0.3, 0.05)): log L, log-det, quadratic form and the per-tile ranks (initial and final), written
 to seed0_n40000_b500_oracle.npz for the full-size GPU parity test.  Takes a few minutes on 8 cores.
 

@@ -4,7 +4,7 @@ Checks: (1) against central finite differences of the CPU oracle's log L (h = 1e
 which equals the frozen-rank derivative when no rank switches inside +-h; tolerance 1e-5 relative);
 (2) against the dense FP64 analytic gradient with a tight fixed rule (eps = 1e-7, ranks up to 57 of the 64
 columns a tile holds), where the TLR factor is close to the dense one (tolerance 1e-4 relative, the
-north-star bound; the oracle's own distance is 3e-5); (3) with the adaptive rule the
+target bound; the oracle's own distance is 3e-5); (3) with the adaptive rule the
 distance to the dense gradient is the compression's own effect (reported, bounded by 5e-2); (4) the torch
 autograd bridge returns the same gradient."""
 import os

@@ -82,7 +82,7 @@ def test_n10000_goldens(tlrgp, goldens):
 
 @pytest.mark.parametrize("theta", [(1.0, 0.3, 0.05), (0.5, 0.6, 0.01), (2.0, 0.1, 0.1), (1.3, 0.15, 0.02)])
 def test_b500_vs_oracle(tlrgp, theta):
-    """Tile size 500 (the north-star tile), N=10,000 seed-0 field: ranks of every tile and log L against the
+    """Tile size 500 (the benchmark tile), N=10,000 seed-0 field: ranks of every tile and log L against the
     oracle, over the corners of the benchmark's theta box (ranks up to ~50 exercise the wide sketch and the
     widest recompression class)."""
     import tlr_oracle as O
@@ -129,7 +129,7 @@ def test_n40000_b250_golden(tlrgp, goldens):
 
 
 def test_n40000_b500_vs_oracle(tlrgp):
-    """The north-star configuration (N=40,000, b=500, adaptive 0.9) at theta=(1, 0.3, 0.05) on the seed-0
+    """The benchmark configuration (N=40,000, b=500, adaptive 0.9) at theta=(1, 0.3, 0.05) on the seed-0
     field: every initial and final tile rank and log L / log-det / quadratic form against the oracle's
     stored result (tests/golden/make_seed0_oracle.py)."""
     path = os.path.join(GOLDEN, "seed0_n40000_b500_oracle.npz")
