@@ -390,6 +390,17 @@ __global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
   __shared__ double Us[16][64];
   __shared__ double zv[64];
   const int tr = tid / 16, tc = tid % 16;
+  // the thread's 4 x 4 entries of A, loaded first so that their latency overlaps the products
+  double av[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int p = pb + tr * 4 + a;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int qq = qb + tc * 4 + q;
+      av[a][q] = (p < b && qq < b && qq <= p) ? A[p + (int64_t)qq * b] : 0.0;
+    }
+  }
   double acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -423,7 +434,7 @@ __global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int qq = qb + tc * 4 + q;
-      if (qq < b && qq <= p) A[p + (int64_t)qq * b] -= acc[a][q];
+      if (qq < b && qq <= p) A[p + (int64_t)qq * b] = av[a][q] - acc[a][q];
     }
   }
   if (blockIdx.x != 0) return;
@@ -509,12 +520,27 @@ __global__ __launch_bounds__(256) void k_recomp_list(SlotArgs sa, int k, int npa
   // to an 8-wave class, whose panel (32 columns) also holds its r_jk <= 16 new columns
   const int c = max(recomp_class(r2), r1 + r2 > 32 ? 2 : 0);
   const int heavy_w[RCLASSES] = {16, 32, 48, 80};
-  const bool panel = j == k + 1;
-  int* cnt = (panel ? sa.wlp_count : sa.wl_count) + WL_STRIDE * (RCLASSES * k + c);
-  const int64_t cap = panel ? sa.wlp_cap : sa.wl_cap;
-  int2* list = (panel ? sa.wlp : sa.wl) + (int64_t)c * cap;
-  if (r1 + r2 > heavy_w[c]) list[atomicAdd(cnt, 1)] = make_int2(q, slot);
-  else list[cap - 1 - atomicAdd(cnt + 1, 1)] = make_int2(q, slot);
+  const int key = (j == k + 1 ? 8 : 0) + 2 * c + (r1 + r2 > heavy_w[c] ? 1 : 0);
+  // one counter update per wave and list (panel or not, class, heavy or light): the wave's entries of a list
+  // take consecutive positions after its leader's atomic add
+  const int lane = lane_id();
+  for (int kk = 0; kk < 16; ++kk) {
+    const unsigned long long mask = __ballot(key == kk);
+    if (mask == 0ull) continue;
+    const int leader = __ffsll((long long)mask) - 1;
+    const bool panel = kk >= 8, heavy = kk & 1;
+    const int cls = (kk >> 1) & 3;
+    int* cnt = (panel ? sa.wlp_count : sa.wl_count) + WL_STRIDE * (RCLASSES * k + cls) + (heavy ? 0 : 1);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(cnt, __popcll(mask));
+    base = __shfl(base, leader);
+    if (key == kk) {
+      const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+      const int64_t cap = panel ? sa.wlp_cap : sa.wl_cap;
+      int2* list = (panel ? sa.wlp : sa.wl) + (int64_t)cls * cap;
+      list[heavy ? pos : cap - 1 - pos] = make_int2(q, slot);
+    }
+  }
 }
 
 // One pair (i, j) = pair index q of step k, for slot `slot`; scr is the work group's scratch.
