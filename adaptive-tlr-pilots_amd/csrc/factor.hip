@@ -389,12 +389,14 @@ __global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
   __shared__ double Xs[16][64];
   __shared__ double Us[16][64];
   __shared__ double zv[64];
-  const int tr = tid / 16, tc = tid % 16;
-  // the thread's 4 x 4 entries of A, loaded first so that their latency overlaps the products
+  // thread (tr, tc) owns rows pb + tr + 16 a and columns qb + 4 tc + q (a, q < 4): for each (a, q) a wave
+  // touches 4 columns x 16 consecutive rows (128-byte runs).  Its 4 x 4 entries of A are loaded first so
+  // that their latency overlaps the products.
+  const int tr = tid % 16, tc = tid / 16;
   double av[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const int p = pb + tr * 4 + a;
+    const int p = pb + tr + 16 * a;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int qq = qb + tc * 4 + q;
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
     for (int c = 0; c < cn; ++c) {
       double xv[4], uv[4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) xv[a] = Xs[c][tr * 4 + a];
+      for (int a = 0; a < 4; ++a) xv[a] = Xs[c][tr + 16 * a];
 #pragma unroll
       for (int q = 0; q < 4; ++q) uv[q] = Us[c][tc * 4 + q];
 #pragma unroll
@@ -429,7 +431,7 @@ __global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const int p = pb + tr * 4 + a;
+    const int p = pb + tr + 16 * a;
     if (p >= b) continue;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
