@@ -721,11 +721,11 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
 }
 
 // Persistent grid over the class-cls work list of step k (grid = resident work groups of the class).
-// The narrow classes ask for 4 (r_jk <= 8) and 2 (r_jk <= 16) waves per SIMD so that the register
-// allocation leaves room for as many work groups as their LDS footprint allows.
+// The 4-wave classes ask for 2 waves per SIMD (two work groups per CU): their code needs about 200-240
+// VGPRs, and a tighter target only adds scratch spills without reaching a third work group.
 template <int NW, int CPW>
 struct RecompWaves {
-  static constexpr int value = NW * CPW <= 8 ? 4 : (NW * CPW <= 16 ? 2 : 1);
+  static constexpr int value = NW * CPW <= 16 ? 2 : 1;
 };
 
 // Entries are claimed one at a time from the list's claim counter, in list order (heavy pairs first), so a
