@@ -80,7 +80,9 @@ int tlr_loglik_batch(tlr_ctx* ctx, int32_t m, const double* thetas, int32_t rule
  * tile after the initial compression and after every recompression are those chosen at theta.  The
  * sigma2 component is exact (0.5 * quad - N / 2: tiles and budgets scale together with sigma2); the range
  * and eta components are central differences in log theta with step h (1e-5 recommended), each
- * evaluation replaying the recorded ranks.  grad is m x 3 (NaN when a perturbed evaluation is not PD).
+ * evaluation replaying the recorded ranks (the eta evaluations reuse theta's compressed tiles, the range
+ * evaluations start their compression sketch from them).  grad is m x 3 (NaN when a perturbed evaluation
+ * is not PD).
  * No counterpart in the reference, which has no gradient (its standard errors use finite differences of
  * log L with re-chosen ranks, codes/15_pilot_standard_errors.py:21-31). */
 int tlr_loglik_grad_batch(tlr_ctx* ctx, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
