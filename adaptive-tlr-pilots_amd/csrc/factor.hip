@@ -379,65 +379,91 @@ __global__ __launch_bounds__(256) void k_diag_update(SlotArgs sa, int k) {
   const double* Xi = P.Xd + (int64_t)i * b * sa.rank_cap;
   double* A = P.D + (int64_t)i * b * b;
   const int tid = threadIdx.x;
-  int e = blockIdx.x;
-  int I = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-  while ((I + 1) * (I + 2) / 2 <= e) ++I;
-  while (I * (I + 1) / 2 > e) --I;
-  const int J = e - I * (I + 1) / 2;
-  const int pb = 64 * I, qb = 64 * J;
   // columns staged 16 at a time (16 KB of LDS, so that several work groups share a CU)
   __shared__ double Xs[16][64];
   __shared__ double Us[16][64];
   __shared__ double zv[64];
-  // thread (tr, tc) owns rows pb + tr + 16 a and columns qb + 4 tc + q (a, q < 4): for each (a, q) a wave
-  // touches 4 columns x 16 consecutive rows (128-byte runs).  Its 4 x 4 entries of A are loaded first so
-  // that their latency overlaps the products.
+  // 64 x 64 blocks (I >= J) of the lower triangle, blockIdx.x, blockIdx.x + gridDim.x, ...; thread (tr, tc)
+  // owns rows pb + tr + 16 a and columns qb + 4 tc + q (a, q < 4): for each (a, q) a wave touches 4 columns
+  // x 16 consecutive rows (128-byte runs).  The next block's 4 x 4 entries of A are loaded before the
+  // current block's products, so that their latency overlaps them.
   const int tr = tid % 16, tc = tid / 16;
+  const int nbt = (b + 63) / 64, nblk = nbt * (nbt + 1) / 2;
+  auto block_of = [](int e, int* pb, int* qb) {
+    int I = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+    while ((I + 1) * (I + 2) / 2 <= e) ++I;
+    while (I * (I + 1) / 2 > e) --I;
+    *pb = 64 * I;
+    *qb = 64 * (e - I * (I + 1) / 2);
+  };
+  auto load_a = [&](int pb, int qb, double (&v)[4][4]) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int p = pb + tr + 16 * a;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int qq = qb + tc * 4 + q;
+        v[a][q] = (p < b && qq < b && qq <= p) ? A[p + (int64_t)qq * b] : 0.0;
+      }
+    }
+  };
+  int pb = 0, qb = 0;
   double av[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int p = pb + tr + 16 * a;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int qq = qb + tc * 4 + q;
-      av[a][q] = (p < b && qq < b && qq <= p) ? A[p + (int64_t)qq * b] : 0.0;
-    }
+  if ((int)blockIdx.x < nblk) {
+    block_of(blockIdx.x, &pb, &qb);
+    load_a(pb, qb, av);
   }
-  double acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
-  for (int c0 = 0; c0 < r; c0 += 16) {
-    const int cn = min(16, r - c0);
-    if (c0 > 0) __syncthreads();
-    for (int e2 = tid; e2 < 64 * cn; e2 += 256) {
-      const int rr = e2 % 64, c = e2 / 64;
-      Xs[c][rr] = (pb + rr < b) ? Xi[pb + rr + (int64_t)(c0 + c) * b] : 0.0;
-      Us[c][rr] = (qb + rr < b) ? U[qb + rr + (int64_t)(c0 + c) * b] : 0.0;
+  for (int e = blockIdx.x; e < nblk; e += gridDim.x) {
+    const int en = e + (int)gridDim.x;
+    int pbn = 0, qbn = 0;
+    double avn[4][4];
+    if (en < nblk) {
+      block_of(en, &pbn, &qbn);
+      load_a(pbn, qbn, avn);
     }
-    __syncthreads();
-    for (int c = 0; c < cn; ++c) {
-      double xv[4], uv[4];
+    double acc[4][4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) xv[a] = Xs[c][tr + 16 * a];
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) uv[q] = Us[c][tc * 4 + q];
+      for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+    for (int c0 = 0; c0 < r; c0 += 16) {
+      const int cn = min(16, r - c0);
+      if (c0 > 0) __syncthreads();
+      for (int e2 = tid; e2 < 64 * cn; e2 += 256) {
+        const int rr = e2 % 64, c = e2 / 64;
+        Xs[c][rr] = (pb + rr < b) ? Xi[pb + rr + (int64_t)(c0 + c) * b] : 0.0;
+        Us[c][rr] = (qb + rr < b) ? U[qb + rr + (int64_t)(c0 + c) * b] : 0.0;
+      }
+      __syncthreads();
+      for (int c = 0; c < cn; ++c) {
+        double xv[4], uv[4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < 4; ++a) xv[a] = Xs[c][tr + 16 * a];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[a][q] = fma(xv[a], uv[q], acc[a][q]);
+        for (int q = 0; q < 4; ++q) uv[q] = Us[c][tc * 4 + q];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[a][q] = fma(xv[a], uv[q], acc[a][q]);
+      }
     }
-  }
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int p = pb + tr + 16 * a;
-    if (p >= b) continue;
+    for (int a = 0; a < 4; ++a) {
+      const int p = pb + tr + 16 * a;
+      if (p >= b) continue;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int qq = qb + tc * 4 + q;
-      if (qq < b && qq <= p) A[p + (int64_t)qq * b] = av[a][q] - acc[a][q];
+      for (int q = 0; q < 4; ++q) {
+        const int qq = qb + tc * 4 + q;
+        if (qq < b && qq <= p) A[p + (int64_t)qq * b] = av[a][q] - acc[a][q];
+      }
     }
+    __syncthreads();   // Xs and Us are restaged for the next block
+    pb = pbn;
+    qb = qbn;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) av[a][q] = avn[a][q];
   }
   if (blockIdx.x != 0) return;
   // forward-solve update of rhs_i
@@ -799,8 +825,11 @@ hipError_t launch_trsm(const SlotArgs& sa, int k, int nslot, hipStream_t s) {
 hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t s) {
   const int rows = sa.nB - k - 1;
   if (rows <= 0) return hipSuccess;
+  // work groups per tile: each takes every DIAG_WG-th 64 x 64 block of the tile's lower triangle
+  constexpr int DIAG_WG = 9;
   const int nb64 = (sa.b + 63) / 64;
-  hipLaunchKernelGGL(k_diag_update, dim3(nb64 * (nb64 + 1) / 2, rows, nslot), dim3(256), 0, s, sa, k);
+  const int g = min(DIAG_WG, nb64 * (nb64 + 1) / 2);
+  hipLaunchKernelGGL(k_diag_update, dim3(g, rows, nslot), dim3(256), 0, s, sa, k);
   return hipGetLastError();
 }
 
