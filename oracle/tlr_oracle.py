@@ -29,7 +29,8 @@ ties to even, saturation, IEEE half/single conversions) and its precision choice
 within 10, stored values within 0.5 % of the reference's run at N = 10k): its rank and precision decisions
 feed each other, so implementation details of the FP16/FP8 conversion move single decisions and with
 them the rest of the factorization.  The analytic dense gradient is pinned by central finite
-differences of ``loglik_dense``.
+differences of ``loglik_dense``.  ``simulate_seed0`` (the benchmark's synthetic field; the reference has no
+such generator) is parity unpinned.
 """
 import numpy as np
 from scipy.linalg import cho_factor, cho_solve, cholesky, solve_triangular, LinAlgError
@@ -477,21 +478,3 @@ def skill(yp, mu, sig):
                 cover90=float((z <= 1.6448536).mean()))
 
 
-def flops_factorization(ranks0, ranksF, b, nB):
-    """Algorithmic flop count of one TLR factorization + forward solve from realised ranks.
-
-    POTRF b^3/3 per diagonal tile; TRSM b^2 r per panel tile; diagonal SYRK 2 b r^2 + 2 b^2 r; every
-    recompression 4 b r_ik r_jk + 8 b w^2 + 22 w^3 + 4 b w r' with w = r_ij + r_jk (the final rank
-    stands in for the intermediate r'); forward solve nB b^2 + 2 sum b r."""
-    f = nB * b ** 3 / 3.0
-    rF = ranksF
-    for k in range(nB):
-        col = rF[k + 1:, k]
-        f += (b * b * col).sum() + (2 * b * col ** 2 + 2 * b * b * col).sum()
-        for i in range(k + 1, nB):
-            for j in range(k + 1, i):
-                if rF[i, k] and rF[j, k]:
-                    w = rF[i, j] + rF[j, k]
-                    f += 4 * b * rF[i, k] * rF[j, k] + 8 * b * w * w + 22 * w ** 3 + 4 * b * w * rF[i, j]
-    f += nB * b * b + 2 * b * rF.sum()
-    return float(f)
