@@ -41,6 +41,7 @@ namespace {
 constexpr int K1 = 64, K2 = 96;     // sketch widths of the two compression rounds
 constexpr int PMIN = 24;            // spare sketch columns required to accept a rank
 constexpr int NPRODUCT = 4;         // products with T, T^T, T, T^T before the rank decision
+constexpr int NCS = 3;              // streams that compress slots concurrently
 constexpr int RANK_CAP = 64;        // column capacity of the U, V slots
 constexpr int WCAP = 128;           // widest recompression concatenation
 
@@ -64,8 +65,10 @@ struct tlr_ctx {
   int kcap = 0;
   hipStream_t stream = nullptr;
   double *sx = nullptr, *sy = nullptr, *y = nullptr, *omega = nullptr;
-  double *bufA = nullptr, *bufB = nullptr, *bufR = nullptr, *part = nullptr;
-  int* need = nullptr;
+  // compression workspace, one set per compression stream (the slots of a batch are compressed in turn on
+  // the main stream and the two side streams)
+  double *bufA[NCS] = {}, *bufB[NCS] = {}, *bufR[NCS] = {}, *part[NCS] = {};
+  int* need[NCS] = {};
   std::vector<void*> allocs;
   std::vector<Slot> slots;
   SlotPtrs* d_slots = nullptr;
@@ -245,11 +248,13 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
   ACHK(hipMemcpy(c->omega, om.data(), om.size() * sizeof(double), hipMemcpyHostToDevice));
   const int64_t nt = c->ntile > 0 ? c->ntile : 1;
   const int nbb = (b + 63) / 64;
-  ACHK(c->alloc(&c->bufA, (size_t)nt * b * c->kcap));
-  ACHK(c->alloc(&c->bufB, (size_t)nt * b * c->kcap));
-  ACHK(c->alloc(&c->bufR, (size_t)nt * c->kcap * c->kcap));
-  ACHK(c->alloc(&c->part, (size_t)nt * nbb * nbb * 2));
-  ACHK(c->alloc(&c->need, (size_t)nt));
+  for (int q = 0; q < NCS; ++q) {
+    ACHK(c->alloc(&c->bufA[q], (size_t)nt * b * c->kcap));
+    ACHK(c->alloc(&c->bufB[q], (size_t)nt * b * c->kcap));
+    ACHK(c->alloc(&c->bufR[q], (size_t)nt * c->kcap * c->kcap));
+    ACHK(c->alloc(&c->part[q], (size_t)nt * nbb * nbb * 2));
+    ACHK(c->alloc(&c->need[q], (size_t)nt));
+  }
   // recompression: a persistent grid per width class sized to the resident capacity of the device; each
   // resident work group owns scratch for Q2v (b x 64), the right blocks of R_u, R_v (128 x 64 each) and
   // the Jacobi core and rotations (128 x 128 each) when they do not fit in LDS, W and the projection
@@ -365,14 +370,20 @@ int tlr_ctx_set_y(tlr_ctx* ctx, const double* y) {
 // warm >= 0 (a replay whose ranks are given, forced != nullptr): the sketch starts from the right factors of
 // stash record `warm` (the recorded evaluation at a nearby theta), and two products (T X, T^T Q) replace the
 // four of a cold start.
+// cs: the compression stream (and workspace set) this slot uses.
 static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternParams& mp, int rule_fixed,
-                         double thr_rel, double thr_abs, const int* forced, int mixed, int warm = -1) {
-  hipStream_t s = c->stream;
+                         double thr_rel, double thr_abs, const int* forced, int mixed, int warm, int cs) {
+  hipStream_t s = c->cls_stream[cs];
+  double* const bufA = c->bufA[cs];
+  double* const bufB = c->bufB[cs];
+  double* const bufR = c->bufR[cs];
+  double* const part = c->part[cs];
+  int* const need = c->need[cs];
   const int b = c->b;
   const int64_t nt = c->ntile;
   HIPCHK(launch_gen_diag(c->sx, c->sy, c->n, b, c->nB, mp, p.D, s));
   if (nt == 0) return TLR_OK;
-  HIPCHK(launch_fill_int(c->need, nt, 1, s));
+  HIPCHK(launch_fill_int(need, nt, 1, s));
   const int64_t ms = (int64_t)b * c->kcap, rs = (int64_t)c->kcap * c->kcap;
   const int kmax = c->kcap;
   const int widths[2] = {K1 < kmax ? K1 : kmax, kmax};
@@ -380,20 +391,20 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
   for (int rd = 0; rd < rounds; ++rd) {
     const int k = widths[rd];
     // alternate products: the buffer holding the latest product and the orthonormal basis of the other side
-    double* cur = c->bufA;   // receives T * X
-    double* oth = c->bufB;   // receives T^T * X
+    double* cur = bufA;   // receives T * X
+    double* oth = bufB;   // receives T^T * X
     const double gemm_flops = 2.0 * b * (double)b * k;     // one product with a b x b tile
     const double qr_flops = 4.0 * b * (double)k * k;        // Householder QR + explicit Q
     TimedLaunch tl{nullptr, nullptr, slot, rd, gemm_flops};
     const bool warm_start = warm >= 0 && forced;
     if (warm_start)
       HIPCHK(launch_warm_sketch(c->stash, c->d_stash_off + (int64_t)warm * nt, c->rep_rank0 + (int64_t)warm * nt,
-                                c->omega, oth, ms, b, k, c->need, nt, s));
+                                c->omega, oth, ms, b, k, need, nt, s));
     bool on = c->tic(1, tl);
     if (warm_start)
-      HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, 0, oth, ms, cur, ms, k, c->need, s));
+      HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, 0, oth, ms, cur, ms, k, need, s));
     else
-      HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, 0, c->omega, 0, cur, ms, k, c->need, s));
+      HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, 0, c->omega, 0, cur, ms, k, need, s));
     c->toc(on, tl);
     int last_op = 0;
     const int nproduct = warm_start ? 2 : NPRODUCT;
@@ -402,11 +413,11 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
       double* dst = (last_op == 0) ? oth : cur;
       TimedLaunch tq{nullptr, nullptr, slot, rd, qr_flops};
       on = c->tic(3, tq);
-      HIPCHK(launch_panel_qr(src, ms, b, k, nullptr, 0, 0, c->need, nt, s));
+      HIPCHK(launch_panel_qr(src, ms, b, k, nullptr, 0, 0, need, nt, s));
       c->toc(on, tq);
       last_op ^= 1;
       on = c->tic(1, tl);
-      HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, last_op, src, ms, dst, ms, k, c->need, s));
+      HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, last_op, src, ms, dst, ms, k, need, s));
       c->toc(on, tl);
     }
     // last product M; the other buffer holds the orthonormal basis Qprev
@@ -418,17 +429,17 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
     if (!forced) {   // the residual only serves the rank choice; a replay's ranks are given
       TimedLaunch tr{nullptr, nullptr, slot, rd, gemm_flops};
       on = c->tic(2, tr);
-      HIPCHK(launch_residual(c->sx, c->sy, c->n, b, nt, mp, Arow, Bcol, ms, k, c->part, c->need, s));
+      HIPCHK(launch_residual(c->sx, c->sy, c->n, b, nt, mp, Arow, Bcol, ms, k, part, need, s));
       c->toc(on, tr);
     }
     TimedLaunch tq{nullptr, nullptr, slot, rd, qr_flops};
     on = c->tic(3, tq);
-    HIPCHK(launch_panel_qr(M, ms, b, k, c->bufR, rs, last_op == 1 ? 1 : 0, c->need, nt, s));
+    HIPCHK(launch_panel_qr(M, ms, b, k, bufR, rs, last_op == 1 ? 1 : 0, need, nt, s));
     c->toc(on, tq);
     const double* Qrow = (last_op == 0) ? M : Qprev;
     const double* Qcol = (last_op == 0) ? Qprev : M;
-    HIPCHK(launch_rank_decide(c->bufR, rs, Qrow, Qcol, ms, c->part, b, k, kmax, PMIN, rule_fixed, thr_rel, thr_abs,
-                              p.U, p.V, (int64_t)b * RANK_CAP, RANK_CAP, p.rank, c->need, c->d_retry + slot,
+    HIPCHK(launch_rank_decide(bufR, rs, Qrow, Qcol, ms, part, b, k, kmax, PMIN, rule_fixed, thr_rel, thr_abs,
+                              p.U, p.V, (int64_t)b * RANK_CAP, RANK_CAP, p.rank, need, c->d_retry + slot,
                               p.status, forced, mixed, nt, s));
   }
   HIPCHK(launch_copy_int(p.rank0, p.rank, nt, s));
@@ -582,6 +593,23 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
   for (int s = 0; s < m; ++s) c->h_replay[s] = replay ? replay[s] : -1;
   HIPCHK(hipMemcpyAsync(c->d_replay, c->h_replay, sizeof(int) * m, hipMemcpyHostToDevice, st));
   const double bmax = (double)(c->n < c->b ? c->n : c->b);
+  // per-slot state on the main stream first; then the slots are compressed in turn on the main stream and
+  // the two side streams (each with its own workspace), which overlap one slot's latency-bound kernels
+  // (panel QR, rank decision) and launch tails with another's.  When a compression kernel is being timed
+  // (its events are recorded on the main stream) every slot stays on the main stream.
+  for (int s = 0; s < m; ++s) {
+    const SlotPtrs& p = c->slots[s].p;
+    HIPCHK(launch_fill_int(p.status, ST_COUNT, 0, st));
+    HIPCHK(launch_fill_int(p.status + ST_FAILED, 1, 0x7fffffff, st));
+    HIPCHK(hipMemcpyAsync(p.rhs, c->y, c->n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (mixed) HIPCHK(launch_fill_int(p.prec, c->ntile, 64, st));
+  }
+  const int ncs = (c->timing_kernel >= 1 && c->timing_kernel <= 3) ? 1 : NCS;
+  if (ncs > 1) {
+    HIPCHK(hipEventRecord(c->ev_list, st));
+    for (int q = 1; q < ncs; ++q) HIPCHK(hipStreamWaitEvent(c->cls_stream[q], c->ev_list, 0));
+  }
+  int ncomp = 0;   // slots compressed so far (stream of the next: ncomp % ncs)
   for (int s = 0; s < m; ++s) {
     const double* th = thetas + 3 * s;
     const SlotPtrs& p = c->slots[s].p;
@@ -590,21 +618,24 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
     mp.inv_range_sqrt3 = sqrt(3.0) / th[1];
     mp.nugget = th[0] * th[2];
     const double thr_abs = rule_fixed ? 0.0 : value * bmax * th[0] * th[2] / (double)c->n;
-    HIPCHK(launch_fill_int(p.status, ST_COUNT, 0, st));
-    HIPCHK(launch_fill_int(p.status + ST_FAILED, 1, 0x7fffffff, st));
-    HIPCHK(hipMemcpyAsync(p.rhs, c->y, c->n * sizeof(double), hipMemcpyDeviceToDevice, st));
     const int* forced = (replay && replay[s] >= 0) ? c->rep_rank0 + (int64_t)replay[s] * c->ntile : nullptr;
-    if (mixed) HIPCHK(launch_fill_int(p.prec, c->ntile, 64, st));
     if (reuse && reuse[s] >= 0) {
       const int rec = reuse[s];
       HIPCHK(launch_gen_diag(c->sx, c->sy, c->n, c->b, c->nB, mp, p.D, st));
       HIPCHK(launch_stash(p.U, p.V, (int64_t)c->b * RANK_CAP, c->b, c->rep_rank0 + (int64_t)rec * c->ntile,
                           c->d_stash_off + (int64_t)rec * c->ntile, c->stash, p.rank, p.rank0, c->ntile, 1, st));
     } else {
-      int rc = compress_slot(c, s, p, mp, rule_fixed, value, thr_abs, forced, mixed, warm ? warm[s] : -1);
+      int rc = compress_slot(c, s, p, mp, rule_fixed, value, thr_abs, forced, mixed, warm ? warm[s] : -1,
+                             ncomp % ncs);
       if (rc != TLR_OK) return rc;
+      ++ncomp;
     }
   }
+  if (ncs > 1)
+    for (int q = 1; q < ncs; ++q) {   // join the side streams into the main stream
+      HIPCHK(hipEventRecord(c->ev_cls[q], c->cls_stream[q]));
+      HIPCHK(hipStreamWaitEvent(st, c->ev_cls[q], 0));
+    }
   if (stash) {
     const int rc = stash_compressed(c, m);
     if (rc != TLR_OK) return rc;
