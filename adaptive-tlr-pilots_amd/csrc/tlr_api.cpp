@@ -54,6 +54,7 @@ struct TimedLaunch {
   hipEvent_t a, b;
   int slot, round;
   double flops_per_tile;   // algorithmic flops per processed tile
+  double bytes_per_tile;   // algorithmic HBM bytes per processed tile
 };
 
 }  // namespace
@@ -69,6 +70,7 @@ struct tlr_ctx {
   // the main stream and the two side streams)
   double *bufA[NCS] = {}, *bufB[NCS] = {}, *bufR[NCS] = {}, *part[NCS] = {};
   int* need[NCS] = {};
+  int nset = 0;                        // workspace sets allocated: min(NCS, max_batch)
   std::vector<void*> allocs;
   std::vector<Slot> slots;
   SlotPtrs* d_slots = nullptr;
@@ -109,7 +111,7 @@ struct tlr_ctx {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   std::vector<TimedLaunch> timed;
-  double timed_ms = 0.0, timed_flops = 0.0;
+  double timed_ms = 0.0, timed_flops = 0.0, timed_bytes = 0.0;
   int64_t timed_launches = 0;
   ~tlr_ctx() {
     if (pred_buf) (void)hipFree(pred_buf);
@@ -248,7 +250,9 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
   ACHK(hipMemcpy(c->omega, om.data(), om.size() * sizeof(double), hipMemcpyHostToDevice));
   const int64_t nt = c->ntile > 0 ? c->ntile : 1;
   const int nbb = (b + 63) / 64;
-  for (int q = 0; q < NCS; ++q) {
+  // one compression workspace per slot that can be compressed concurrently (at most NCS)
+  c->nset = max_batch < NCS ? max_batch : NCS;
+  for (int q = 0; q < c->nset; ++q) {
     ACHK(c->alloc(&c->bufA[q], (size_t)nt * b * c->kcap));
     ACHK(c->alloc(&c->bufB[q], (size_t)nt * b * c->kcap));
     ACHK(c->alloc(&c->bufR[q], (size_t)nt * c->kcap * c->kcap));
@@ -395,7 +399,11 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
     double* oth = bufB;   // receives T^T * X
     const double gemm_flops = 2.0 * b * (double)b * k;     // one product with a b x b tile
     const double qr_flops = 4.0 * b * (double)k * k;        // Householder QR + explicit Q
-    TimedLaunch tl{nullptr, nullptr, slot, rd, gemm_flops};
+    // bytes per tile: a product reads the b x k operand and the tile's 2 x 2b coordinates and writes the
+    // b x k result (the tile itself is regenerated, never stored); the residual reads two b x k panels; the
+    // QR reads and writes the panel
+    const double gemm_bytes = 16.0 * b * k + 32.0 * b, qr_bytes = 16.0 * b * (double)k;
+    TimedLaunch tl{nullptr, nullptr, slot, rd, gemm_flops, gemm_bytes};
     const bool warm_start = warm >= 0 && forced;
     if (warm_start)
       HIPCHK(launch_warm_sketch(c->stash, c->d_stash_off + (int64_t)warm * nt, c->rep_rank0 + (int64_t)warm * nt,
@@ -411,7 +419,7 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
     for (int pidx = 1; pidx < nproduct; ++pidx) {
       double* src = (last_op == 0) ? cur : oth;
       double* dst = (last_op == 0) ? oth : cur;
-      TimedLaunch tq{nullptr, nullptr, slot, rd, qr_flops};
+      TimedLaunch tq{nullptr, nullptr, slot, rd, qr_flops, qr_bytes};
       on = c->tic(3, tq);
       HIPCHK(launch_panel_qr(src, ms, b, k, nullptr, 0, 0, need, nt, s));
       c->toc(on, tq);
@@ -427,12 +435,12 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
     const double* Arow = (last_op == 0) ? M : Qprev;
     const double* Bcol = (last_op == 0) ? Qprev : M;
     if (!forced) {   // the residual only serves the rank choice; a replay's ranks are given
-      TimedLaunch tr{nullptr, nullptr, slot, rd, gemm_flops};
+      TimedLaunch tr{nullptr, nullptr, slot, rd, gemm_flops, gemm_bytes};
       on = c->tic(2, tr);
       HIPCHK(launch_residual(c->sx, c->sy, c->n, b, nt, mp, Arow, Bcol, ms, k, part, need, s));
       c->toc(on, tr);
     }
-    TimedLaunch tq{nullptr, nullptr, slot, rd, qr_flops};
+    TimedLaunch tq{nullptr, nullptr, slot, rd, qr_flops, qr_bytes};
     on = c->tic(3, tq);
     HIPCHK(launch_panel_qr(M, ms, b, k, bufR, rs, last_op == 1 ? 1 : 0, need, nt, s));
     c->toc(on, tq);
@@ -461,7 +469,7 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
 // lists of step k are complete before POTRF can record a failure.
 static hipError_t launch_step_lookahead(tlr_ctx* c, const SlotArgs& sa, int k, int nslot) {
   hipError_t e;
-  TimedLaunch tt{nullptr, nullptr, -1, k, 0.0};
+  TimedLaunch tt{nullptr, nullptr, -1, k, 0.0, 0.0};
   const bool on = c->tic(5, tt);
   if ((e = launch_recomp_list(sa, k, nslot, c->stream)) != hipSuccess) return e;
   if ((e = hipEventRecord(c->ev_list, c->stream)) != hipSuccess) return e;
@@ -497,18 +505,25 @@ static hipError_t launch_step_lookahead(tlr_ctx* c, const SlotArgs& sa, int k, i
   return hipSuccess;
 }
 
-// Algorithmic flops of every recompression of one evaluation, replayed on the host from the initial ranks
-// and the rank log: pair (i, j) of step k with r_ik, r_jk > 0 costs 4 b r_ik r_jk (W and U_ik W) +
-// 8 b w^2 (QR of both concatenations) + 22 w^3 (core SVD) + 4 b w r' (rebuilt factors), w = r_ij + r_jk,
-// r' the rank it was truncated to.  Steps from failed_at on are skipped (they did not run).
-static double recompress_flops(const tlr_ctx* c, const std::vector<int>& rank0, const std::vector<int>& rfin,
-                               const std::vector<int>& rlog, int failed_at) {
+// Algorithmic work of the factorization steps that run with lookahead (k = 0 .. nB - 3, the launch group
+// that timing class 5 brackets), replayed on the host from the initial ranks and the rank log.  Per step k:
+//  * recompressions: pair (i, j), k < j < i, with r_ik, r_jk > 0 costs 4 b r_ik r_jk (W and U_ik W) +
+//    8 b w^2 (QR of both concatenations) + 22 w^3 (core SVD) + 4 b w r' (rebuilt factors) flops, w = r_ij + r_jk,
+//    r' the rank it was truncated to, and reads U, V of the tiles (i, j), (i, k), (j, k) and writes the new
+//    (i, j) pair: 2 b (r_ij + r_ik + r_jk + r') doubles;
+//  * the lookahead of step k + 1: POTRF b^3 / 3 and the explicit inverse b^3 / 3 flops (reading and writing
+//    the tile: 2 b^2 doubles each); TRSM b^2 r per tile (i, k+1) (V read and written: 2 b r doubles); the
+//    diagonal update 2 b r^2 + 2 b^2 r flops per tile (the lower triangle read and written: b (b + 1)
+//    doubles).
+// Steps from failed_at on are skipped (they did not run).
+static void step_work(const tlr_ctx* c, const std::vector<int>& rank0, const std::vector<int>& rfin,
+                      const std::vector<int>& rlog, int failed_at, double* flops, double* bytes) {
   std::vector<int> cur(rank0);
   const double b = (double)c->b;
-  double fl = 0.0;
+  double fl = 0.0, by = 0.0;
   int64_t off = 0;
-  const int kend = failed_at >= 0 && failed_at < c->nB ? failed_at : c->nB;
-  for (int k = 0; k < kend; ++k) {
+  const int kend = failed_at >= 0 && failed_at < c->nB ? failed_at - 1 : c->nB;
+  for (int k = 0; k < kend && c->nB - k - 1 >= 2; ++k) {
     const int mk = c->nB - k - 1;
     int64_t q = 0;
     for (int a = 1; a < mk; ++a)
@@ -519,11 +534,21 @@ static double recompress_flops(const tlr_ctx* c, const std::vector<int>& rank0, 
         const int64_t tij = tri_index(i, j);
         const double w = (double)(cur[tij] + rjk), rn = (double)rlog[off + q];
         fl += 4.0 * b * rik * rjk + 8.0 * b * w * w + 22.0 * w * w * w + 4.0 * b * w * rn;
+        by += 16.0 * b * (cur[tij] + rik + rjk + rn);
         cur[tij] = rlog[off + q];
       }
     off += (int64_t)mk * (mk - 1) / 2;
+    const int k1 = k + 1;
+    fl += 2.0 * b * b * b / 3.0;
+    by += 4.0 * 8.0 * b * b;
+    for (int i = k1 + 1; i < c->nB; ++i) {
+      const double r = (double)rfin[tri_index(i, k1)];
+      fl += b * b * r + 2.0 * b * r * r + 2.0 * b * b * r;
+      by += 16.0 * b * r + (r > 0.0 ? 8.0 * b * (b + 1.0) : 0.0);
+    }
   }
-  return fl;
+  *flops += fl;
+  *bytes += by;
 }
 
 // Copy the initially compressed factors of slots 0..m-1 into stash records 0..m-1 (compact, offsets from
@@ -560,9 +585,9 @@ static int stash_compressed(tlr_ctx* c, int m) {
 // reuse[s] >= 0 makes it also take record reuse[s]'s stashed compressed factors instead of compressing
 // (valid when theta differs from the record's only in eta, which leaves the off-diagonal tiles unchanged);
 // stash != 0 stores every slot's compressed factors as records 0..m-1.
-static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
-                     const int* replay, double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma,
-                     int32_t* ranks_factor, const int* reuse = nullptr, int stash = 0, const int* warm = nullptr) {
+static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value,
+                          int32_t precision, const int* replay, double* out, int32_t* status, int32_t* failed_at,
+                          int32_t* ranks_sigma, int32_t* ranks_factor, const int* reuse, int stash, const int* warm) {
   if (!c || !thetas || !out || m < 1 || m > c->max_batch) return set_err(TLR_ERR_ARG, "invalid argument");
   if (rule != TLR_RULE_FIXED && rule != TLR_RULE_ADAPTIVE && rule != TLR_RULE_MIXED)
     return set_err(TLR_ERR_ARG, "unknown rule");
@@ -604,7 +629,8 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
     HIPCHK(hipMemcpyAsync(p.rhs, c->y, c->n * sizeof(double), hipMemcpyDeviceToDevice, st));
     if (mixed) HIPCHK(launch_fill_int(p.prec, c->ntile, 64, st));
   }
-  const int ncs = (c->timing_kernel >= 1 && c->timing_kernel <= 3) ? 1 : NCS;
+  int ncs = (c->timing_kernel >= 1 && c->timing_kernel <= 3) ? 1 : c->nset;
+  if (ncs > m) ncs = m;
   if (ncs > 1) {
     HIPCHK(hipEventRecord(c->ev_list, st));
     for (int q = 1; q < ncs; ++q) HIPCHK(hipStreamWaitEvent(c->cls_stream[q], c->ev_list, 0));
@@ -670,6 +696,7 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
       if (c->timing_kernel != 5) {
         const double tiles = t.round == 0 ? (double)c->ntile : (double)retry[t.slot];
         c->timed_flops += tiles * t.flops_per_tile;
+        c->timed_bytes += tiles * t.bytes_per_tile;
       }
     }
     c->timed.clear();
@@ -683,7 +710,7 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
         HIPCHK(hipMemcpy(rf.data(), p.rank, c->ntile * sizeof(int), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(lg.data(), p.rlog, c->nlog * sizeof(int), hipMemcpyDeviceToHost));
         const int fa = stv[ST_FAILED] == 0x7fffffff ? -1 : stv[ST_FAILED];
-        c->timed_flops += recompress_flops(c, r0, rf, lg, fa);
+        step_work(c, r0, rf, lg, fa, &c->timed_flops, &c->timed_bytes);
       }
     }
   }
@@ -721,17 +748,34 @@ static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, 
   return ret;
 }
 
+// run_batch_impl, and on an error return the side streams and the main stream are drained, so that the next
+// call finds no work in flight (it reuses the pinned staging buffers and the work lists).
+static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
+                     const int* replay, double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma,
+                     int32_t* ranks_factor, const int* reuse = nullptr, int stash = 0, const int* warm = nullptr) {
+  const int rc = run_batch_impl(c, m, thetas, rule, value, precision, replay, out, status, failed_at, ranks_sigma,
+                                ranks_factor, reuse, stash, warm);
+  if (rc < 0 && c) {
+    for (int q = 1; q < RCLASSES; ++q)
+      if (c->cls_stream[q]) (void)hipStreamSynchronize(c->cls_stream[q]);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+  }
+  return rc;
+}
+
 int tlr_ctx_set_timing(tlr_ctx* ctx, int32_t kernel_class) {
   if (!ctx || kernel_class < 0 || kernel_class > 5) return set_err(TLR_ERR_ARG, "invalid kernel class");
   ctx->timing_kernel = kernel_class;
   ctx->timed_ms = 0.0;
   ctx->timed_flops = 0.0;
+  ctx->timed_bytes = 0.0;
   ctx->timed_launches = 0;
   return TLR_OK;
 }
 
-int tlr_ctx_get_timing(const tlr_ctx* ctx, double* total_ms, int64_t* launches, double* flops) {
+int tlr_ctx_get_timing(const tlr_ctx* ctx, double* total_ms, int64_t* launches, double* flops, double* bytes) {
   if (!ctx) return set_err(TLR_ERR_ARG, "null context");
+  if (bytes) *bytes = ctx->timed_bytes;
   if (total_ms) *total_ms = ctx->timed_ms;
   if (launches) *launches = ctx->timed_launches;
   if (flops) *flops = ctx->timed_flops;

@@ -36,7 +36,7 @@ SIGNATURES = {
     "tlr_solve": (ctypes.c_int, [_VP, _D, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _D, ctypes.c_int64,
                                  ctypes.c_int64, ctypes.c_int32, _D]),
     "tlr_ctx_set_timing": (ctypes.c_int, [_VP, ctypes.c_int32]),
-    "tlr_ctx_get_timing": (ctypes.c_int, [_VP, _D, _I64, _D]),
+    "tlr_ctx_get_timing": (ctypes.c_int, [_VP, _D, _I64, _D, _D]),
     "tlr_ctx_precisions": (ctypes.c_int, [_VP, ctypes.c_int32, _I]),
     "tlr_last_error": (ctypes.c_char_p, []),
     "tlr_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32]),

@@ -137,16 +137,18 @@ class TLRContext:
 
     def set_timing(self, kernel_class):
         """Bracket launches of one kernel class with HIP events (1 Matern products, 2 residual, 3 panel QR,
-        5 recompressions of each factorization step, 0 off); resets the accumulators."""
+        5 factorization steps with lookahead, 0 off); resets the accumulators."""
         check(lib.tlr_ctx_set_timing(self._ptr, int(kernel_class)))
 
     def get_timing(self):
-        """(total kernel ms, launches, algorithmic flops) since set_timing."""
+        """(total kernel ms, launches, algorithmic flops, algorithmic HBM bytes) since set_timing."""
         ms = _lib.ctypes.c_double()
         n = _lib.ctypes.c_int64()
         fl = _lib.ctypes.c_double()
-        check(lib.tlr_ctx_get_timing(self._ptr, _lib.ctypes.byref(ms), _lib.ctypes.byref(n), _lib.ctypes.byref(fl)))
-        return ms.value, n.value, fl.value
+        by = _lib.ctypes.c_double()
+        check(lib.tlr_ctx_get_timing(self._ptr, _lib.ctypes.byref(ms), _lib.ctypes.byref(n), _lib.ctypes.byref(fl),
+                                     _lib.ctypes.byref(by)))
+        return ms.value, n.value, fl.value, by.value
 
     def set_y(self, y):
         check(lib.tlr_ctx_set_y(self._ptr, dptr(f64(y, (self.n,)))))

@@ -133,11 +133,11 @@ def main():
         r = step()
         print(f"# step {s}: {time.perf_counter() - t0:.3f} s", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
-    kms, launches, flops = ctx.get_timing()
+    kms, launches, flops, kbytes = ctx.get_timing()
     # one more (untimed for throughput) pass timing the Matern products of the compression
     ctx.set_timing(1)
     step()
-    pms, plaunches, pflops = ctx.get_timing()
+    pms, plaunches, pflops, pbytes = ctx.get_timing()
     ctx.set_timing(0)
     ms_step = 1e3 * elapsed / a.steps
     evals = a.batch * a.steps / elapsed

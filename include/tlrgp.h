@@ -108,13 +108,16 @@ int tlr_solve(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, i
 
 /* Timing of one kernel class with HIP events on the context's stream, for roofline reports (no
  * counterpart in the reference, which times whole evaluations with time.time(), 17:359-362).
- * kernel_class: 0 off, 1 Matern products, 2 residual, 3 panel QR, 5 the recompressions of each
- * factorization step (work lists and the four width classes, from the first launch to the join).  Setting
- * it resets the accumulators; tlr_ctx_get_timing returns the summed time, the number of timed launches
- * (steps for class 5) and their algorithmic flops (products: 2 b^2 k per processed tile; recompression:
- * 4 b r_ik r_jk + 8 b w^2 + 22 w^3 + 4 b w r' per pair, w = r_ij + r_jk, from the rank logs). */
+ * kernel_class: 0 off, 1 Matern products, 2 residual, 3 panel QR, 5 one factorization step with lookahead
+ * (the work lists and recompressions of step k with the POTRF, inverse, TRSM and diagonal update of step
+ * k + 1 that overlap them, from the first launch to the join of the streams).  Setting it resets the
+ * accumulators; tlr_ctx_get_timing returns the summed time, the number of timed launches (steps for class
+ * 5), their algorithmic flops and their algorithmic HBM bytes (products: 2 b^2 k flops and 16 b k + 32 b
+ * bytes per processed tile; class 5: per recompression pair 4 b r_ik r_jk + 8 b w^2 + 22 w^3 + 4 b w r'
+ * flops and 16 b (r_ij + r_ik + r_jk + r') bytes, w = r_ij + r_jk, from the rank logs, plus POTRF and the
+ * inverse 2 b^3 / 3, TRSM b^2 r and the diagonal update 2 b r^2 + 2 b^2 r per tile). */
 int tlr_ctx_set_timing(tlr_ctx* ctx, int32_t kernel_class);
-int tlr_ctx_get_timing(const tlr_ctx* ctx, double* total_ms, int64_t* launches, double* flops);
+int tlr_ctx_get_timing(const tlr_ctx* ctx, double* total_ms, int64_t* launches, double* flops, double* bytes);
 
 /* Storage precision (8, 16, 32 or 64 bits) of every factor tile of the last evaluation in batch slot
  * `slot` under TLR_RULE_MIXED: prec is nB x nB int32 row-major, entry [i * nB + j] for the tile (i, j),

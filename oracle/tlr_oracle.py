@@ -30,7 +30,11 @@ within 10, stored values within 0.5 % of the reference's run at N = 10k): its ra
 feed each other, so implementation details of the FP16/FP8 conversion move single decisions and with
 them the rest of the factorization.  The analytic dense gradient is pinned by central finite
 differences of ``loglik_dense``.  ``simulate_seed0`` (the benchmark's synthetic field; the reference has no
-such generator) is parity unpinned.
+such generator) is pinned by its own definition: C^{-1} y gives back the PCG64 normal stream drawn after the
+sites, which are Morton-sorted, on fresh draws and on the committed N = 10k / 40k fields;
+``cholesky_blocked`` against LAPACK's Cholesky, ``cov_full_lowmem`` entry for entry against ``cov_full``,
+``tlr_matvec_L`` as the inverse of ``tlr_forward``, ``loglik_dense_parts`` against the reference's dense
+quadratic form (``bound_terms_n10000.json#z2``).
 """
 import numpy as np
 from scipy.linalg import cho_factor, cho_solve, cholesky, solve_triangular, LinAlgError

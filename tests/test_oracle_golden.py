@@ -5,7 +5,7 @@ import os
 
 import numpy as np
 import pytest
-from scipy.linalg import cholesky
+from scipy.linalg import cholesky, solve_triangular
 
 import tlr_oracle as O
 from conftest import GOLDEN
@@ -171,6 +171,85 @@ def test_n10000_predict_skill_matches_reference(goldens):
     ref = [r for r in goldens["tlr_cholesky"]["10000"]["rules"] if r["rule"] == "adaptive"][0]["skill_true_theta"]
     for key in ("rmse", "crps", "cover90"):
         assert abs(sk[key] - ref[key]) < 1e-9, (key, sk[key], ref[key])
+
+
+# ------------------------------------------------------------------ generator and dense helpers
+def test_cov_full_lowmem_equals_cov_full():
+    """The row-block builder is the same matrix entry for entry (block not dividing N)."""
+    S = np.random.default_rng(11).random((700, 2))
+    th = (1.7, 0.21, 0.03)
+    np.testing.assert_array_equal(O.cov_full_lowmem(S, th, block=256), O.cov_full(S, th))
+
+
+def test_cholesky_blocked_matches_scipy():
+    """Blocked right-looking Cholesky (several blocks, the last one short) against LAPACK's."""
+    S = np.random.default_rng(12).random((900, 2))
+    K = O.cov_full(S, (1.0, 0.3, 0.05))
+    L = O.cholesky_blocked(K.copy(), block=200)
+    C = cholesky(K, lower=True)
+    assert np.abs(L - C).max() <= 1e-12 * np.abs(C).max()
+    assert not np.triu(L, 1).any()
+
+
+def _seed0_stream(n, seed):
+    """Sites and the normal draws of simulate_seed0's PCG64 stream: random((n, 2)) then standard_normal(n)."""
+    rng = np.random.default_rng(seed)
+    S0 = rng.random((n, 2))
+    return S0[O.morton_order(S0)], rng.standard_normal(n)
+
+
+def test_simulate_seed0_is_cholesky_of_the_normal_stream():
+    """y = C z with C the dense Cholesky factor on the Morton-ordered sites and z the PCG64 normals drawn
+    after the sites: so C^{-1} y must give back that stream, and the sites must be Morton-sorted."""
+    n = 1500
+    S, y = O.simulate_seed0(n, seed=3)
+    S_want, z_want = _seed0_stream(n, 3)
+    np.testing.assert_array_equal(S, S_want)
+    assert np.array_equal(O.morton_order(S), np.arange(n))
+    C = cholesky(O.cov_full(S, (1.0, 0.3, 0.05)), lower=True)
+    z = solve_triangular(C, y, lower=True)
+    assert np.abs(z - z_want).max() < 1e-10
+
+
+@pytest.mark.parametrize("n", [10000, 40000])
+def test_committed_seed0_fields(n):
+    """The benchmark fields (N = 10k: dense branch; 40k: blocked branch) are simulate_seed0's output: sites
+    bit for bit, and the leading m rows of y reproduce the first m normals (a Cholesky factor's leading
+    m x m block is the factor of the leading block of K, and y[:m] = C[:m, :m] z[:m])."""
+    a = np.load(os.path.join(GOLDEN, f"seed0_n{n}.npy"))
+    S, y = a[:, :2], a[:, 2]
+    S_want, z_want = _seed0_stream(n, 0)
+    np.testing.assert_array_equal(S, S_want)
+    m = 3000
+    C = cholesky(O.cov_full(S[:m], (1.0, 0.3, 0.05)), lower=True)
+    z = solve_triangular(C, y[:m], lower=True)
+    assert np.abs(z - z_want[:m]).max() < 1e-9
+
+
+def test_tlr_matvec_L_inverts_the_forward_solve():
+    """L (L^{-1} B) = B for the tile factor, and L^T (L^{-T} Z) = Z through the dense assembly of L."""
+    S, y = O.simulate_seed0(800, seed=4)
+    th = (1.0, 0.3, 0.05)
+    bnd = O.tile_bounds(800, 200)
+    ll, fac, info = O.tlr_loglik(S, y, th, bnd, "adaptive", 0.9)
+    assert info["completed"]
+    B = np.random.default_rng(5).standard_normal((800, 3))
+    Z = O.tlr_forward(*fac, bnd, B)
+    np.testing.assert_allclose(O.tlr_matvec_L(*fac, bnd, Z), B, rtol=0, atol=1e-10)
+    L = O.tlr_matvec_L(*fac, bnd, np.eye(800))
+    assert not np.triu(L, 1).any()
+    X = O.tlr_backward(*fac, bnd, B)
+    np.testing.assert_allclose(L.T @ X, B, rtol=0, atol=1e-10)
+
+
+@pytest.mark.slow
+def test_dense_parts_match_bound_terms(goldens):
+    """Dense quadratic form y^T K^{-1} y and log L of the reference field against bound_terms_n10000.json."""
+    S, y = _golden_field(10000)
+    parts = O.loglik_dense_parts(S, y, (1.0, 0.3, 0.05))
+    g = goldens["bound_terms"]["10000"]
+    assert abs(parts["quad"] - g["z2"]) < 1e-9 * g["z2"], (parts["quad"], g["z2"])
+    assert abs(parts["loglik"] - g["dense_loglik"]) < 1e-9 * abs(g["dense_loglik"])
 
 
 # ------------------------------------------------------------------ storage-precision rule (mixed)
