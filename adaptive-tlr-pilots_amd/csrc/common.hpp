@@ -44,6 +44,26 @@ __device__ __forceinline__ double group_sum_dpp(double v) {
   return v;
 }
 
+// The same for a float (one 32-bit move per exchange).
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+template <int TP>
+__device__ __forceinline__ float group_sum_dpp(float v) {
+  static_assert(TP == 2 || TP == 4 || TP == 8 || TP == 16, "group of 2, 4, 8 or 16 lanes");
+  v += dpp_move<0xB1>(v);
+  if (TP >= 4) v += dpp_move<0x4E>(v);
+  if (TP >= 8) v += dpp_move<0x141>(v);
+  if (TP >= 16) v += dpp_move<0x140>(v);
+  return v;
+}
+
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 __device__ __forceinline__ double readlane_d(double v, int lane) {
   const long long x = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)(x & 0xffffffffll), lane);
@@ -56,6 +76,32 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
 __device__ __forceinline__ double wave_sum(double v) {
   v = group_sum_dpp<16>(v);
   return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+  v = group_sum_dpp<16>(v);
+  return (readlane_f(v, 0) + readlane_f(v, 16)) + (readlane_f(v, 32) + readlane_f(v, 48));
+}
+
+// Arithmetic precision of the factorization's recompressions: unit roundoff of the working type (the
+// Jacobi tolerance and noise floor scale with it), and the rounding of product operands to bfloat16
+// (round to nearest even on the upper 16 bits of the float; NaN and infinity kept) for the BF16 mode, whose
+// products then accumulate in float.
+template <class T> struct Prec;
+template <> struct Prec<double> {
+  static constexpr double eps = 2.220446049250313e-16;    // 2^-52
+  static constexpr double tiny2 = 4.930380657631324e-32;   // eps^2
+};
+template <> struct Prec<float> {
+  static constexpr float eps = 1.1920928955078125e-07f;    // 2^-23
+  static constexpr float tiny2 = 1.4210854715202004e-14f;  // eps^2
+};
+
+__device__ __forceinline__ float round_bf16(float x) {
+  unsigned u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return x;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return __uint_as_float(u & 0xffff0000u);
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (TLR_WAVE - 1); }

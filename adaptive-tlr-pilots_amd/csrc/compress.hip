@@ -60,8 +60,8 @@ __global__ __launch_bounds__(256, 4) void k_matern_product(const double* __restr
                                                            int64_t n, int b, MaternParams mp, int op,
                                                            const double* __restrict__ X, int64_t xstride,
                                                            double* __restrict__ out, int64_t ostride, int k,
-                                                           const int* __restrict__ active) {
-  const int64_t t = blockIdx.z;
+                                                           const int* __restrict__ active, int64_t t0) {
+  const int64_t t = t0 + blockIdx.z;
   if (active && !active[t]) return;
   int i, j;
   tri_decode(t, &i, &j);
@@ -161,8 +161,8 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ sx,
                                                   int64_t n, int b, MaternParams mp,
                                                   const double* __restrict__ A, const double* __restrict__ B,
                                                   int64_t stride, int k, double* __restrict__ part,
-                                                  const int* __restrict__ active) {
-  const int64_t t = blockIdx.z;
+                                                  const int* __restrict__ active, int64_t t0) {
+  const int64_t t = t0 + blockIdx.z;
   if (active && !active[t]) return;
   int i, j;
   tri_decode(t, &i, &j);
@@ -348,8 +348,8 @@ __global__ void k_copy_int(int* dst, const int* src, int64_t count) {
 // sum of r0.  dir 0: slot -> stash; dir 1: stash -> slot, and rank[t] = rank0[t] = r0[t].
 __global__ void k_stash(double* __restrict__ U, double* __restrict__ V, int64_t fstride, int b,
                         const int* __restrict__ r0, const int64_t* __restrict__ off, double* __restrict__ stash,
-                        int* __restrict__ rank, int* __restrict__ rank0, int dir) {
-  const int64_t t = blockIdx.y;
+                        int* __restrict__ rank, int* __restrict__ rank0, int dir, int64_t t0) {
+  const int64_t t = t0 + blockIdx.y;
   const int r = r0[t];
   double* st = stash + 2 * (int64_t)b * off[t];
   double* Ut = U + t * fstride;
@@ -372,8 +372,8 @@ __global__ void k_stash(double* __restrict__ U, double* __restrict__ V, int64_t 
 // dominant range of T up to the change in theta (the right factor V_t spans the recorded tile's row space).
 __global__ void k_warm_sketch(const double* __restrict__ stash, const int64_t* __restrict__ off,
                              const int* __restrict__ r0, const double* __restrict__ omega, double* __restrict__ X,
-                             int64_t xstride, int b, int k, const int* __restrict__ active) {
-  const int64_t t = blockIdx.y;
+                             int64_t xstride, int b, int k, const int* __restrict__ active, int64_t t0) {
+  const int64_t t = t0 + blockIdx.y;
   if (active && !active[t]) return;
   const int r = min(r0[t], k);
   const int64_t cnt = (int64_t)b * r0[t];
@@ -388,20 +388,27 @@ __global__ void k_warm_sketch(const double* __restrict__ stash, const int64_t* _
 
 // ------------------------------------------------------------------------------------ launchers
 static int nblk(int64_t a, int64_t d) { return (int)((a + d - 1) / d); }
+// Grids whose y or z dimension runs over tiles are launched in chunks of at most this many tiles (the
+// limit of those grid dimensions), each with its first tile as an argument.
+constexpr int64_t TILE_CHUNK = 65535;
 
 hipError_t launch_stash(double* U, double* V, int64_t fstride, int b, const int* r0, const int64_t* off,
                         double* stash, int* rank, int* rank0, int64_t ntile, int dir, hipStream_t s) {
-  if (ntile <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_stash, dim3(4, (unsigned)ntile), dim3(256), 0, s, U, V, fstride, b, r0, off, stash, rank, rank0,
-                     dir);
+  for (int64_t t0 = 0; t0 < ntile; t0 += TILE_CHUNK) {
+    const int64_t nt = ntile - t0 < TILE_CHUNK ? ntile - t0 : TILE_CHUNK;
+    hipLaunchKernelGGL(k_stash, dim3(4, (unsigned)nt), dim3(256), 0, s, U, V, fstride, b, r0, off, stash, rank, rank0,
+                       dir, t0);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_warm_sketch(const double* stash, const int64_t* off, const int* r0, const double* omega, double* X,
                               int64_t xstride, int b, int k, const int* active, int64_t ntile, hipStream_t s) {
-  if (ntile <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_warm_sketch, dim3(nblk((int64_t)b * k, 256 * 8), (unsigned)ntile), dim3(256), 0, s, stash, off,
-                     r0, omega, X, xstride, b, k, active);
+  for (int64_t t0 = 0; t0 < ntile; t0 += TILE_CHUNK) {
+    const int64_t nt = ntile - t0 < TILE_CHUNK ? ntile - t0 : TILE_CHUNK;
+    hipLaunchKernelGGL(k_warm_sketch, dim3(nblk((int64_t)b * k, 256 * 8), (unsigned)nt), dim3(256), 0, s, stash, off,
+                       r0, omega, X, xstride, b, k, active, t0);
+  }
   return hipGetLastError();
 }
 
@@ -427,9 +434,12 @@ hipError_t launch_gen_diag(const double* sx, const double* sy, int64_t n, int b,
 hipError_t launch_matern_product(const double* sx, const double* sy, int64_t n, int b, int64_t ntile, MaternParams mp,
                                  int op, const double* X, int64_t xstride, double* out, int64_t ostride, int k,
                                  const int* active, hipStream_t s) {
-  dim3 grid(nblk(b, MM_ROWS), nblk(k, MM_COLS), (unsigned)ntile);
-  hipLaunchKernelGGL(k_matern_product, grid, dim3(256), 0, s, sx, sy, n, b, mp, op, X, xstride, out, ostride, k,
-                     active);
+  for (int64_t t0 = 0; t0 < ntile; t0 += TILE_CHUNK) {
+    const int64_t nt = ntile - t0 < TILE_CHUNK ? ntile - t0 : TILE_CHUNK;
+    dim3 grid(nblk(b, MM_ROWS), nblk(k, MM_COLS), (unsigned)nt);
+    hipLaunchKernelGGL(k_matern_product, grid, dim3(256), 0, s, sx, sy, n, b, mp, op, X, xstride, out, ostride, k,
+                       active, t0);
+  }
   return hipGetLastError();
 }
 
@@ -464,8 +474,11 @@ hipError_t launch_residual(const double* sx, const double* sy, int64_t n, int b,
                            const double* A, const double* B, int64_t stride, int k, double* part, const int* active,
                            hipStream_t s) {
   const int nbb = nblk(b, 64);
-  dim3 grid(nbb, nbb, (unsigned)ntile);
-  hipLaunchKernelGGL(k_residual, grid, dim3(256), 0, s, sx, sy, n, b, mp, A, B, stride, k, part, active);
+  for (int64_t t0 = 0; t0 < ntile; t0 += TILE_CHUNK) {
+    const int64_t nt = ntile - t0 < TILE_CHUNK ? ntile - t0 : TILE_CHUNK;
+    dim3 grid(nbb, nbb, (unsigned)nt);
+    hipLaunchKernelGGL(k_residual, grid, dim3(256), 0, s, sx, sy, n, b, mp, A, B, stride, k, part, active, t0);
+  }
   return hipGetLastError();
 }
 

@@ -9,6 +9,8 @@
 // The forward solve is folded into the factorization (right-looking): z_k = L_kk^{-1} rhs_k at step k,
 // then rhs_i -= U_ik (V_ik^T z_k) for i > k; it uses the same final factors as the reference's
 // left-looking solve.  Updates with r_ik = 0 or r_jk = 0 are skipped, as in the reference.
+#include <type_traits>
+
 #include "common.hpp"
 #include "panel.hpp"
 #include "round.hpp"
@@ -571,10 +573,22 @@ __global__ __launch_bounds__(256) void k_recomp_list(SlotArgs sa, int k, int npa
   }
 }
 
-// One pair (i, j) = pair index q of step k, for slot `slot`; scr is the work group's scratch.
-template <int RPL, int NW, int CPW>
-__device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q, int slot, double* scr) {
+// Append pair q of slot `slot` to the redo list (step k, panel, level) (level 0: FP64, 1: FP32).
+__device__ __forceinline__ void push_redo(const SlotArgs& sa, int k, int panel, int level, int q, int slot) {
+  int* cnt = sa.wlr_count + WL_STRIDE * ((k * 2 + panel) * 2 + level);
+  const int pos = atomicAdd(cnt, 1);
+  if (pos < sa.wlr_cap) sa.wlr[(int64_t)(panel * 2 + level) * sa.wlr_cap + pos] = make_int2(q, slot);
+  else atomicMax(&sa.slot(slot).status[ST_GUARD], 3);
+}
+
+// One pair (i, j) = pair index q of step k, for slot `slot`; scr is the work group's scratch.  PM is the
+// arithmetic (working type T: double for PM_FP64, float otherwise; PM_BF16 rounds the operands of the two
+// products W = V_ik^T V_jk and U_ik W to bfloat16).  panel: the pair came from a panel list (j = k + 1).
+template <int RPL, int NW, int CPW, int PM>
+__device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q, int slot, double* scr, int panel) {
   using L = RecompLayout<NW, CPW>;
+  using T = typename std::conditional<PM == PM_FP64, double, float>::type;
+  constexpr bool BF = PM == PM_BF16;
   const SlotPtrs P = sa.slot(slot);
   const int b = sa.b;
   int a_, b_;
@@ -584,24 +598,38 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
   const int r1 = P.rank[tij], rik = P.rank[tik], r2 = P.rank[tjk];
   const int w = r1 + r2;
   const double thr_rel = sa.thr[2 * slot], thr_abs = sa.thr[2 * slot + 1];
-  extern __shared__ double lds[];
+  const int64_t lo = sa.rlog_off[k] + q;
+  const int rep = sa.replay[slot];
+  if constexpr (PM != PM_FP64) {
+    // a replay keeps the arithmetic its record used for this pair (frozen precisions)
+    if (rep >= 0) {
+      const int pr = sa.rep_rpl[(int64_t)rep * sa.nlog + lo];
+      if (pr >= 0 && pr < PM) {
+        if (threadIdx.x == 0) push_redo(sa, k, panel, pr == PM_FP32 ? 1 : 0, q, slot);
+        return;
+      }
+    }
+  }
+  extern __shared__ double lds_raw[];
+  T* lds = reinterpret_cast<T*>(lds_raw);
+  T* scrT = reinterpret_cast<T*>(scr);
   // rik x r2 (phases 1-3) and the r1 x r2 temporary (phases 2-3): LDS, or scratch past the other regions
-  double* Wm = L::WC_LDS ? lds : scr + (int64_t)b * 64 + 2 * 128 * 64 + 2 * 128 * 128;
-  double* Ct = L::WC_LDS ? lds + 64 * L::R2MAX : Wm + 64 * 64;
-  double* Sb = lds;                           // RB x w staging     (phase 5, aliases the front)
-  double* Q2v = scr;                          // b x r2
-  double* RvR = L::RR_LDS ? lds + L::OFF_RV : scr + (int64_t)b * 64;            // w x r2: [Cv; R2v]
-  double* RuR = L::RR_LDS ? lds + L::OFF_RU : scr + (int64_t)b * 64 + 128 * 64; // w x r2: [Cu; R2u]
-  double* sc = lds + L::OFF_SC;               // 1 / sigma_a of U_ij
-  double* sg = lds + L::OFF_SG;               // sigma_a
+  T* Wm = L::WC_LDS ? lds : scrT + (int64_t)b * 64 + 2 * 128 * 64 + 2 * 128 * 128;
+  T* Ct = L::WC_LDS ? lds + 64 * L::R2MAX : Wm + 64 * 64;
+  T* Sb = lds;                                // RB x w staging     (phase 5, aliases the front)
+  T* Q2v = scrT;                              // b x r2
+  T* RvR = L::RR_LDS ? lds + L::OFF_RV : scrT + (int64_t)b * 64;            // w x r2: [Cv; R2v]
+  T* RuR = L::RR_LDS ? lds + L::OFF_RU : scrT + (int64_t)b * 64 + 128 * 64; // w x r2: [Cu; R2u]
+  T* sc = lds + L::OFF_SC;                    // 1 / sigma_a of U_ij
+  T* sg = lds + L::OFF_SG;                    // sigma_a
   const bool core_lds = w <= L::WJ;
-  double* G = core_lds ? lds + L::OFF_G : scr + (int64_t)b * 64 + 2 * 128 * 64;
-  double* J = core_lds ? lds + L::OFF_J : G + 128 * 128;
+  T* G = core_lds ? lds + L::OFF_G : scrT + (int64_t)b * 64 + 2 * 128 * 64;
+  T* J = core_lds ? lds + L::OFF_J : G + 128 * 128;
   // Householder vectors of the panel QRs (phases 2-3 only): the core's LDS region, unused until phase 4
   static_assert(2 * 64 * RPL <= L::WJ * L::WJ, "reflector buffer must fit the core region");
-  double* vbuf = lds + L::OFF_G;
-  __shared__ double tau[8 * NW];
-  __shared__ double sig[L::WMAX];
+  T* vbuf = lds + L::OFF_G;
+  __shared__ T tau[8 * NW];
+  __shared__ T sig[L::WMAX];
   __shared__ int perm[L::WMAX];
   __shared__ int flag, s_rank;
   __shared__ double s_tail;
@@ -613,20 +641,28 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
   const double* Vik = P.V + tik * fs;
   const double* Ujk = P.U + tjk * fs;
   const double* Vjk = P.V + tjk * fs;
-  RegPanel<RPL, NW, CPW> pan;
+  RegPanel<RPL, NW, CPW, T> pan;
+  // reduced precision: this thread's share of ||U_jk||_F^2 and ||U_ik W||_F^2 (the guard's update scale)
+  T nu2 = (T)0, nx2 = (T)0;
 
   // ---- 1. W = V_ik^T V_jk
-  pan.load(Vjk, b, b, r2);
-  panel_coef(pan, Vik, b, b, rik, r2, (const double*)nullptr, Wm, rik);
+  pan.template load<BF>(Vjk, b, b, r2);
+  panel_coef<BF>(pan, Vik, b, b, rik, r2, (const T*)nullptr, Wm, rik);
   // ---- 2. V side: [V_ij, U_jk] = [V_ij, Q2v] [[I, Cv], [0, R2v]]
   pan.load(Ujk, b, b, r2);
+  if constexpr (PM != PM_FP64) {
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj)
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) nu2 = fma(pan.a[jj][t], pan.a[jj][t], nu2);
+  }
   if (r1 > 0) {
-    panel_coef(pan, Vij, b, b, r1, r2, (const double*)nullptr, RvR, w);
+    panel_coef(pan, Vij, b, b, r1, r2, (const T*)nullptr, RvR, w);
     __syncthreads();
-    panel_axpy(pan, Vij, b, b, r1, r2, (const double*)nullptr, RvR, w, -1.0);
-    panel_coef(pan, Vij, b, b, r1, r2, (const double*)nullptr, Ct, r1);
+    panel_axpy(pan, Vij, b, b, r1, r2, (const T*)nullptr, RvR, w, (T)-1);
+    panel_coef(pan, Vij, b, b, r1, r2, (const T*)nullptr, Ct, r1);
     __syncthreads();
-    panel_axpy(pan, Vij, b, b, r1, r2, (const double*)nullptr, Ct, r1, -1.0);
+    panel_axpy(pan, Vij, b, b, r1, r2, (const T*)nullptr, Ct, r1, (T)-1);
     for (int e = tid; e < r1 * r2; e += NT) RvR[(e % r1) + (e / r1) * w] += Ct[e];
   }
   pan.geqr2(b, r2, vbuf, tau);
@@ -635,21 +671,27 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
   pan.store(Q2v, b, b, r2);
   // ---- 3. U side: [U_ij, -U_ik W] = [Q1, Q2u] [[D, Cu], [0, R2u]]
   for (int a = wv; a < r1; a += NW) {
-    double d = 0.0;
-    for (int p = ln; p < b; p += 64) { const double u = Uij[p + (int64_t)a * b]; d = fma(u, u, d); }
+    T d = (T)0;
+    for (int p = ln; p < b; p += 64) { const T u = (T)Uij[p + (int64_t)a * b]; d = fma(u, u, d); }
     d = wave_sum(d);
-    if (ln == 0) { const double sgm = sqrt(d); sg[a] = sgm; sc[a] = sgm > 0.0 ? 1.0 / sgm : 0.0; }
+    if (ln == 0) { const T sgm = tsqrt(d); sg[a] = sgm; sc[a] = sgm > (T)0 ? (T)1 / sgm : (T)0; }
   }
   __syncthreads();
   pan.zero();
-  panel_axpy(pan, Uik, b, b, rik, r2, (const double*)nullptr, Wm, rik, -1.0);
+  panel_axpy<BF>(pan, Uik, b, b, rik, r2, (const T*)nullptr, Wm, rik, (T)-1);
+  if constexpr (PM != PM_FP64) {
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj)
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) nx2 = fma(pan.a[jj][t], pan.a[jj][t], nx2);
+  }
   if (r1 > 0) {
     panel_coef(pan, Uij, b, b, r1, r2, sc, RuR, w);
     __syncthreads();
-    panel_axpy(pan, Uij, b, b, r1, r2, sc, RuR, w, -1.0);
+    panel_axpy(pan, Uij, b, b, r1, r2, sc, RuR, w, (T)-1);
     panel_coef(pan, Uij, b, b, r1, r2, sc, Ct, r1);
     __syncthreads();
-    panel_axpy(pan, Uij, b, b, r1, r2, sc, Ct, r1, -1.0);
+    panel_axpy(pan, Uij, b, b, r1, r2, sc, Ct, r1, (T)-1);
     for (int e = tid; e < r1 * r2; e += NT) RuR[(e % r1) + (e / r1) * w] += Ct[e];
   }
   pan.geqr2(b, r2, vbuf, tau);
@@ -657,34 +699,65 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
   pan.org2r(b, r2, vbuf, tau);   // Q2u in registers from here to the outputs (ends with a barrier)
   // ---- 4. core K = R_u R_v^T, stored transposed: the one-sided Jacobi runs on K^T (on these cores it
   // needs fewer sweeps than on K), giving K^T J = V S (columns orthogonal, norms S) and J = U; truncation
+  T nk2 = (T)0;
   for (int e = tid; e < w * w; e += NT) {
     const int a = e % w, c = e / w;
-    double s = (a == c && a < r1) ? sg[a] : 0.0;
+    T s = (a == c && a < r1) ? sg[a] : (T)0;
     for (int m = 0; m < r2; ++m) s = fma(RuR[a + m * w], RvR[c + m * w], s);
     G[c + a * w] = s;
+    if constexpr (PM != PM_FP64) nk2 = fma(s, s, nk2);
   }
-  __syncthreads();
+  if constexpr (PM != PM_FP64) {
+    // the guard: ||core||_F and the update scale ||U_ik W||_F ||U_jk||_F, summed over the work group in
+    // double
+    __shared__ double gsum[3 * NW];
+    __shared__ int s_go;
+    const double a0 = wave_sum((double)nk2), a1 = wave_sum((double)nx2), a2 = wave_sum((double)nu2);
+    if (ln == 0) { gsum[wv] = a0; gsum[NW + wv] = a1; gsum[2 * NW + wv] = a2; }
+    __syncthreads();
+    if (tid == 0) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+      for (int u = 0; u < NW; ++u) { s0 += gsum[u]; s1 += gsum[NW + u]; s2 += gsum[2 * NW + u]; }
+      const double nk = sqrt(s0), scale = fmax(nk, sqrt(s1) * sqrt(s2));
+      const double thr = sa.rule_fixed ? thr_rel * nk : thr_abs;
+      int go = PM;   // arithmetic this pair gets: PM, or a redo level
+      if (!(nk == nk && scale <= 1.7976931348623157e308)) go = PM_FP64;   // non-finite: FP64 redo
+      else if (rep < 0) {
+        if (PM == PM_BF16 && !(thr >= sa.guard * EPS_BF16 * scale)) go = PM_FP32;
+        if (go == PM_FP32 && !(thr >= sa.guard * EPS_FP32 * scale)) go = PM_FP64;
+      }
+      if (go != PM) push_redo(sa, k, panel, go == PM_FP32 ? 1 : 0, q, slot);
+      s_go = go;
+    }
+    __syncthreads();
+    if (s_go != PM) return;
+  } else {
+    __syncthreads();
+  }
   const int sweeps = jacobi_svd_auto<L::JTP>(G, w, J, w, w, sig, perm, &flag, 60);
-  if (sweeps == -2) {   // non-finite core: report, keep the tile unchanged
-    if (tid == 0) atomicMax(&P.status[ST_GUARD], 2);
+  if (sweeps == -2) {   // non-finite core
+    if constexpr (PM == PM_FP64) {   // report, keep the tile unchanged
+      if (tid == 0) atomicMax(&P.status[ST_GUARD], 2);
+    } else {                         // redo in FP64
+      if (tid == 0) push_redo(sa, k, panel, 0, q, slot);
+    }
     return;
   }
   if (tid == 0) {
     double tot = 0.0;
-    for (int c = 0; c < w; ++c) tot += sig[c] * sig[c];
+    for (int c = 0; c < w; ++c) tot += (double)sig[c] * (double)sig[c];
     const double thr = sa.rule_fixed ? thr_rel * sqrt(tot) : thr_abs;
     double acc = 0.0;
     int r = w;
     for (int c = w - 1; c >= 0; --c) {
-      const double s = sig[perm[c]];
+      const double s = (double)sig[perm[c]];
       const double nacc = acc + s * s;
       if (sqrt(nacc) <= thr) { acc = nacc; r = c; } else break;
     }
-    const int64_t lo = sa.rlog_off[k] + q;
-    const int rep = sa.replay[slot];
     if (rep >= 0) r = min(sa.rep_rlog[(int64_t)rep * sa.nlog + lo], w);   // frozen ranks
     if (r > sa.rank_cap) { atomicMax(&P.status[ST_RANKCAP], 1); r = sa.rank_cap; }
     P.rlog[lo] = r;
+    P.rpl[lo] = PM;
     s_rank = r;
     s_tail = sqrt(acc);
   }
@@ -703,45 +776,47 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
       // V_ij <- [V_ij, Q2v] V_core[:, perm]
       for (int e = tid; e < RB * w; e += NT) {
         const int rr = e & (RB - 1), m = e >> rbs, p = r0 + rr;
-        Sb[m * RB + rr] = (p < b) ? (m < r1 ? Vij[p + (int64_t)m * b] : Q2v[p + (int64_t)(m - r1) * b]) : 0.0;
+        Sb[m * RB + rr] = (p < b) ? (m < r1 ? (T)Vij[p + (int64_t)m * b] : Q2v[p + (int64_t)(m - r1) * b]) : (T)0;
       }
       __syncthreads();
       for (int e = tid; e < RB * rn; e += NT) {
         const int rr = e & (RB - 1), c = e >> rbs, p = r0 + rr;
-        const double* Vc = G + perm[c] * w;
-        double s = 0.0;
+        const T* Vc = G + perm[c] * w;
+        T s = (T)0;
         for (int m = 0; m < w; ++m) s = fma(Sb[m * RB + rr], Vc[m], s);
-        const double sv = sig[perm[c]];
-        if (p < b) Vij[p + (int64_t)c * b] = sv > 0.0 ? s / sv : 0.0;
+        const T sv = sig[perm[c]];
+        if (p < b) Vij[p + (int64_t)c * b] = sv > (T)0 ? (double)(s / sv) : 0.0;
       }
       __syncthreads();
       // U_ij <- [Q1, Q2u] (U S)_core[:, perm],  Q1 = U_ij diag(1/sigma)
       for (int e = tid; e < RB * r1; e += NT) {
         const int rr = e & (RB - 1), m = e >> rbs, p = r0 + rr;
-        Sb[m * RB + rr] = (p < b) ? Uij[p + (int64_t)m * b] * sc[m] : 0.0;
+        Sb[m * RB + rr] = (p < b) ? (T)Uij[p + (int64_t)m * b] * sc[m] : (T)0;
       }
       const int rr_own = ln - (r0 - 64 * t);
 #pragma unroll
       for (int jj = 0; jj < CPW; ++jj) {
-        const int c = RegPanel<RPL, NW, CPW>::col_of(wv, jj);
+        const int c = RegPanel<RPL, NW, CPW, T>::col_of(wv, jj);
         if (c < r2 && rr_own >= 0 && rr_own < RB) Sb[(r1 + c) * RB + rr_own] = pan.a[jj][t];
       }
       __syncthreads();
       for (int e = tid; e < RB * rn; e += NT) {
         const int rr = e & (RB - 1), c = e >> rbs, p = r0 + rr;
-        const double* Uc = J + perm[c] * w;
-        double s = 0.0;
+        const T* Uc = J + perm[c] * w;
+        T s = (T)0;
         for (int m = 0; m < w; ++m) s = fma(Sb[m * RB + rr], Uc[m], s);
-        if (p < b) Uij[p + (int64_t)c * b] = s * sig[perm[c]];
+        if (p < b) Uij[p + (int64_t)c * b] = (double)(s * sig[perm[c]]);
       }
       __syncthreads();
     }
   }
-  if (sa.mixed) {
-    // mixed rule: storage precision under the budget left after truncation (17:203-204); the work
-    // group's scratch (Q2v is dead by now) holds the Gram matrices, sig[] the reductions
-    const int bi = (int)min<int64_t>(b, sa.n - (int64_t)i * b), bj = (int)min<int64_t>(b, sa.n - (int64_t)j * b);
-    round_pair_wg(Uij, b, bi, Vij, b, bj, rn, thr_abs - s_tail, scr, sig);
+  if constexpr (PM == PM_FP64) {
+    if (sa.mixed) {
+      // mixed rule: storage precision under the budget left after truncation (17:203-204); the work
+      // group's scratch (Q2v is dead by now) holds the Gram matrices, sig[] the reductions
+      const int bi = (int)min<int64_t>(b, sa.n - (int64_t)i * b), bj = (int)min<int64_t>(b, sa.n - (int64_t)j * b);
+      round_pair_wg(Uij, b, bi, Vij, b, bj, rn, thr_abs - s_tail, scr, sig);
+    }
   }
   if (tid == 0) P.rank[tij] = rn;
 }
@@ -756,14 +831,29 @@ struct RecompWaves {
 
 // Entries are claimed one at a time from the list's claim counter, in list order (heavy pairs first), so a
 // work group that becomes resident late (its CU still held by another kernel) only takes what is left.
-template <int RPL, int NW, int CPW>
+// src 0: the class-cls list of step k (panel or the rest); src 1 / 2: the FP64 / FP32 redo list of step k
+// (pairs that a reduced-precision launch handed on; complete before this launch starts).
+template <int RPL, int NW, int CPW, int PM>
 __global__ __launch_bounds__(64 * NW, (RecompWaves<NW, CPW>::value)) void k_recompress(SlotArgs sa, int k, int cls,
-                                                                                        int panel) {
-  int* cnts = (panel ? sa.wlp_count : sa.wl_count) + WL_STRIDE * (RCLASSES * k + cls);
-  const int64_t cap = panel ? sa.wlp_cap : sa.wl_cap;
-  const int nh = __builtin_amdgcn_readfirstlane(cnts[0]);
-  const int cnt = nh + __builtin_amdgcn_readfirstlane(cnts[1]);
-  const int2* list = (panel ? sa.wlp : sa.wl) + (int64_t)cls * cap;
+                                                                                        int panel, int src) {
+  int* cnts;
+  const int2* list;
+  int64_t cap;
+  int nh, cnt;
+  if (src == 0) {
+    cnts = (panel ? sa.wlp_count : sa.wl_count) + WL_STRIDE * (RCLASSES * k + cls);
+    cap = panel ? sa.wlp_cap : sa.wl_cap;
+    nh = __builtin_amdgcn_readfirstlane(cnts[0]);
+    cnt = nh + __builtin_amdgcn_readfirstlane(cnts[1]);
+    list = (panel ? sa.wlp : sa.wl) + (int64_t)cls * cap;
+  } else {
+    const int lvl = src == 1 ? 0 : 1;
+    cnts = sa.wlr_count + WL_STRIDE * ((k * 2 + panel) * 2 + lvl);
+    cap = sa.wlr_cap;
+    nh = (int)min<int64_t>(__builtin_amdgcn_readfirstlane(cnts[0]), cap);
+    cnt = nh;
+    list = sa.wlr + (int64_t)(panel * 2 + lvl) * cap;
+  }
   double* scr = sa.scr_pool + (int64_t)blockIdx.x * sa.scr_stride;
   __shared__ int s_claim;
   for (;;) {
@@ -773,8 +863,8 @@ __global__ __launch_bounds__(64 * NW, (RecompWaves<NW, CPW>::value)) void k_reco
     __syncthreads();
     if (q >= cnt) break;
     const int2 e = list[q < nh ? q : cap - 1 - (q - nh)];
-    recompress_pair<RPL, NW, CPW>(sa, k, __builtin_amdgcn_readfirstlane(e.x), __builtin_amdgcn_readfirstlane(e.y),
-                                  scr);
+    recompress_pair<RPL, NW, CPW, PM>(sa, k, __builtin_amdgcn_readfirstlane(e.x), __builtin_amdgcn_readfirstlane(e.y),
+                                      scr, panel);
     __syncthreads();
   }
 }
@@ -825,55 +915,64 @@ hipError_t launch_trsm(const SlotArgs& sa, int k, int nslot, hipStream_t s) {
 hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t s) {
   const int rows = sa.nB - k - 1;
   if (rows <= 0) return hipSuccess;
-  // work groups per tile: each takes every DIAG_WG-th 64 x 64 block of the tile's lower triangle
-  constexpr int DIAG_WG = 9;
+  // work groups per tile: each takes every diag_wg-th 64 x 64 block of the tile's lower triangle
   const int nb64 = (sa.b + 63) / 64;
-  const int g = min(DIAG_WG, nb64 * (nb64 + 1) / 2);
+  const int g = max(1, min(sa.diag_wg, nb64 * (nb64 + 1) / 2));
   hipLaunchKernelGGL(k_diag_update, dim3(g, rows, nslot), dim3(256), 0, s, sa, k);
   return hipGetLastError();
 }
 
-template <int RPL, int NW, int CPW>
+template <int PM>
+using RecompT = typename std::conditional<PM == PM_FP64, double, float>::type;
+
+template <int RPL, int NW, int CPW, int PM>
 static hipError_t recomp_attr() {
   static bool attr_set = false;
   hipError_t e = hipSuccess;
   if (!attr_set) {
-    e = hipFuncSetAttribute((const void*)k_recompress<RPL, NW, CPW>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(RecompLayout<NW, CPW>::TOTAL * sizeof(double)));
+    e = hipFuncSetAttribute((const void*)k_recompress<RPL, NW, CPW, PM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(RecompLayout<NW, CPW>::TOTAL * sizeof(RecompT<PM>)));
     attr_set = e == hipSuccess;
   }
   return e;
 }
 
-template <int RPL, int NW, int CPW>
-static hipError_t launch_recomp_t(const SlotArgs& sa, int k, int cls, int panel, int grid, hipStream_t s) {
-  hipError_t e = recomp_attr<RPL, NW, CPW>();
+template <int RPL, int NW, int CPW, int PM>
+static hipError_t launch_recomp_t(const SlotArgs& sa, int k, int cls, int panel, int grid, int src, hipStream_t s) {
+  hipError_t e = recomp_attr<RPL, NW, CPW, PM>();
   if (e != hipSuccess) return e;
-  const size_t lds = (size_t)RecompLayout<NW, CPW>::TOTAL * sizeof(double);
-  hipLaunchKernelGGL((k_recompress<RPL, NW, CPW>), dim3(grid), dim3(64 * NW), lds, s, sa, k, cls, panel);
+  const size_t lds = (size_t)RecompLayout<NW, CPW>::TOTAL * sizeof(RecompT<PM>);
+  hipLaunchKernelGGL((k_recompress<RPL, NW, CPW, PM>), dim3(grid), dim3(64 * NW), lds, s, sa, k, cls, panel, src);
   return hipGetLastError();
 }
 
-template <int RPL, int NW, int CPW>
+template <int RPL, int NW, int CPW, int PM>
 static hipError_t occ_t(int* out) {
-  hipError_t e = recomp_attr<RPL, NW, CPW>();
+  hipError_t e = recomp_attr<RPL, NW, CPW, PM>();
   if (e != hipSuccess) return e;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, (const void*)k_recompress<RPL, NW, CPW>, 64 * NW,
-                                                      RecompLayout<NW, CPW>::TOTAL * sizeof(double));
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, (const void*)k_recompress<RPL, NW, CPW, PM>, 64 * NW,
+                                                      RecompLayout<NW, CPW>::TOTAL * sizeof(RecompT<PM>));
 }
 
-hipError_t recompress_occupancy(int b, int blocks_per_cu[RCLASSES]) {
+template <int PM>
+static hipError_t occupancy_pm(int b, int blocks_per_cu[RCLASSES]) {
   hipError_t e;
   if (b <= 256) {
-    if ((e = occ_t<4, 4, 2>(blocks_per_cu + 0)) != hipSuccess) return e;
-    if ((e = occ_t<4, 4, 4>(blocks_per_cu + 1)) != hipSuccess) return e;
-    if ((e = occ_t<4, 8, 4>(blocks_per_cu + 2)) != hipSuccess) return e;
-    return occ_t<4, 8, 8>(blocks_per_cu + 3);
+    if ((e = occ_t<4, 4, 2, PM>(blocks_per_cu + 0)) != hipSuccess) return e;
+    if ((e = occ_t<4, 4, 4, PM>(blocks_per_cu + 1)) != hipSuccess) return e;
+    if ((e = occ_t<4, 8, 4, PM>(blocks_per_cu + 2)) != hipSuccess) return e;
+    return occ_t<4, 8, 8, PM>(blocks_per_cu + 3);
   }
-  if ((e = occ_t<8, 4, 2>(blocks_per_cu + 0)) != hipSuccess) return e;
-  if ((e = occ_t<8, 4, 4>(blocks_per_cu + 1)) != hipSuccess) return e;
-  if ((e = occ_t<8, 8, 4>(blocks_per_cu + 2)) != hipSuccess) return e;
-  return occ_t<8, 8, 8>(blocks_per_cu + 3);
+  if ((e = occ_t<8, 4, 2, PM>(blocks_per_cu + 0)) != hipSuccess) return e;
+  if ((e = occ_t<8, 4, 4, PM>(blocks_per_cu + 1)) != hipSuccess) return e;
+  if ((e = occ_t<8, 8, 4, PM>(blocks_per_cu + 2)) != hipSuccess) return e;
+  return occ_t<8, 8, 8, PM>(blocks_per_cu + 3);
+}
+
+hipError_t recompress_occupancy(int b, int pm, int blocks_per_cu[RCLASSES]) {
+  if (pm == PM_FP32) return occupancy_pm<PM_FP32>(b, blocks_per_cu);
+  if (pm == PM_BF16) return occupancy_pm<PM_BF16>(b, blocks_per_cu);
+  return occupancy_pm<PM_FP64>(b, blocks_per_cu);
 }
 
 // Work lists of step k (one thread per pair and slot); false when step k has no pairs.
@@ -885,26 +984,35 @@ hipError_t launch_recomp_list(const SlotArgs& sa, int k, int nslot, hipStream_t 
   return hipGetLastError();
 }
 
-// One persistent launch of class cls for step k (grid work groups, each looping over the class list,
-// the panel list when panel != 0; scr is the class's scratch pool, grid x scr_stride doubles).
-hipError_t launch_recomp_class(const SlotArgs& sa_in, int k, int cls, int panel, int grid, double* scr,
-                               hipStream_t s) {
-  SlotArgs sa = sa_in;
-  sa.scr_pool = scr;
+template <int PM>
+static hipError_t launch_class_pm(const SlotArgs& sa, int k, int cls, int panel, int grid, int src, hipStream_t s) {
   if (sa.b <= 256) {
     switch (cls) {
-      case 0: return launch_recomp_t<4, 4, 2>(sa, k, 0, panel, grid, s);
-      case 1: return launch_recomp_t<4, 4, 4>(sa, k, 1, panel, grid, s);
-      case 2: return launch_recomp_t<4, 8, 4>(sa, k, 2, panel, grid, s);
-      default: return launch_recomp_t<4, 8, 8>(sa, k, 3, panel, grid, s);
+      case 0: return launch_recomp_t<4, 4, 2, PM>(sa, k, 0, panel, grid, src, s);
+      case 1: return launch_recomp_t<4, 4, 4, PM>(sa, k, 1, panel, grid, src, s);
+      case 2: return launch_recomp_t<4, 8, 4, PM>(sa, k, 2, panel, grid, src, s);
+      default: return launch_recomp_t<4, 8, 8, PM>(sa, k, 3, panel, grid, src, s);
     }
   }
   switch (cls) {
-    case 0: return launch_recomp_t<8, 4, 2>(sa, k, 0, panel, grid, s);
-    case 1: return launch_recomp_t<8, 4, 4>(sa, k, 1, panel, grid, s);
-    case 2: return launch_recomp_t<8, 8, 4>(sa, k, 2, panel, grid, s);
-    default: return launch_recomp_t<8, 8, 8>(sa, k, 3, panel, grid, s);
+    case 0: return launch_recomp_t<8, 4, 2, PM>(sa, k, 0, panel, grid, src, s);
+    case 1: return launch_recomp_t<8, 4, 4, PM>(sa, k, 1, panel, grid, src, s);
+    case 2: return launch_recomp_t<8, 8, 4, PM>(sa, k, 2, panel, grid, src, s);
+    default: return launch_recomp_t<8, 8, 8, PM>(sa, k, 3, panel, grid, src, s);
   }
+}
+
+// One persistent launch of class cls for step k (grid work groups, each looping over the class list, the
+// panel list when panel != 0, or a redo list (src)); scr is the launch's scratch pool, grid x scr_stride
+// doubles.  A redo list may hold pairs of any class: it runs on the widest class (cls 3).
+hipError_t launch_recomp_class(const SlotArgs& sa_in, int k, int cls, int panel, int grid, double* scr, hipStream_t s,
+                               int pm, int src) {
+  SlotArgs sa = sa_in;
+  sa.scr_pool = scr;
+  if (src != 0) cls = 3;
+  if (pm == PM_FP32) return launch_class_pm<PM_FP32>(sa, k, cls, panel, grid, src, s);
+  if (pm == PM_BF16) return launch_class_pm<PM_BF16>(sa, k, cls, panel, grid, src, s);
+  return launch_class_pm<PM_FP64>(sa, k, cls, panel, grid, src, s);
 }
 
 hipError_t launch_final(const SlotArgs& sa, int nslot, hipStream_t s) {

@@ -5,16 +5,36 @@
 //  * panel_coef / panel_axpy: projections of the panel against a global matrix (block Gram-Schmidt).
 //  * jacobi_svd: one-sided (Hestenes) Jacobi on the columns of a small square matrix, round-robin
 //    ordering, rotations accumulated into J.
+//  Every routine is templated on the working type T (double, or float for the reduced-precision
+//  recompressions); global operands stay double and are converted on load and store.  BF = true rounds the
+//  operands of a product to bfloat16 (products accumulate in T = float).
 #pragma once
 #include "common.hpp"
 
 namespace tlr {
 
-template <int RPL, int NW, int CPW_ = 8>
+__device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float tsqrt(float x) { return __fsqrt_rn(x); }
+
+// x converted to the working type, rounded to bfloat16 when BF (only for T = float)
+template <class T, bool BF = false>
+__device__ __forceinline__ T cvt(double x) {
+  static_assert(!BF || sizeof(T) == 4, "bfloat16 operands need float accumulation");
+  if constexpr (BF) return round_bf16((float)x);
+  else return (T)x;
+}
+template <class T, bool BF = false>
+__device__ __forceinline__ T cvt(float x) {
+  static_assert(!BF || sizeof(T) == 4, "bfloat16 operands need float accumulation");
+  if constexpr (BF) return round_bf16(x);
+  else return (T)x;
+}
+
+template <int RPL, int NW, int CPW_ = 8, class T = double>
 struct RegPanel {
   static constexpr int CPW = CPW_;
   static constexpr int NT = 64 * NW;
-  double a[CPW][RPL];
+  T a[CPW][RPL];
 
   __device__ __forceinline__ static int col_of(int w, int jj) { return w + NW * jj; }
 
@@ -22,11 +42,12 @@ struct RegPanel {
 #pragma unroll
     for (int jj = 0; jj < CPW; ++jj)
 #pragma unroll
-      for (int t = 0; t < RPL; ++t) a[jj][t] = 0.0;
+      for (int t = 0; t < RPL; ++t) a[jj][t] = (T)0;
   }
 
   // A: column-major, leading dimension lda; entries outside m x n read as zero.
-  __device__ __forceinline__ void load(const double* A, int64_t lda, int m, int n) {
+  template <bool BF = false, class S>
+  __device__ __forceinline__ void load(const S* A, int64_t lda, int m, int n) {
     const int w = wave_id(), l = lane_id();
 #pragma unroll
     for (int jj = 0; jj < CPW; ++jj) {
@@ -34,12 +55,13 @@ struct RegPanel {
 #pragma unroll
       for (int t = 0; t < RPL; ++t) {
         const int p = l + 64 * t;
-        a[jj][t] = (j < n && p < m) ? A[p + (int64_t)j * lda] : 0.0;
+        a[jj][t] = (j < n && p < m) ? cvt<T, BF>(A[p + (int64_t)j * lda]) : (T)0;
       }
     }
   }
 
-  __device__ __forceinline__ void store(double* A, int64_t lda, int m, int n) const {
+  template <class S>
+  __device__ __forceinline__ void store(S* A, int64_t lda, int m, int n) const {
     const int w = wave_id(), l = lane_id();
 #pragma unroll
     for (int jj = 0; jj < CPW; ++jj) {
@@ -48,7 +70,7 @@ struct RegPanel {
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
           const int p = l + 64 * t;
-          if (p < m) A[p + (int64_t)j * lda] = a[jj][t];
+          if (p < m) A[p + (int64_t)j * lda] = (S)a[jj][t];
         }
       }
     }
@@ -56,16 +78,16 @@ struct RegPanel {
 
   // Copy column slot jc (wave-uniform) into x.
   // Apply the reflector in vb (rows >= c nonzero) with factor tc to every own column j > c.
-  __device__ __forceinline__ void apply_reflector(int c, int n, const double* vb, double tc) {
+  __device__ __forceinline__ void apply_reflector(int c, int n, const T* vb, T tc) {
     const int w = wave_id(), l = lane_id();
-    double v[RPL];
+    T v[RPL];
 #pragma unroll
     for (int t = 0; t < RPL; ++t) v[t] = vb[l + 64 * t];
 #pragma unroll
     for (int jj = 0; jj < CPW; ++jj) {
       const int j = col_of(w, jj);
       if (j > c && j < n) {
-        double d = 0.0;
+        T d = (T)0;
 #pragma unroll
         for (int t = 0; t < RPL; ++t) d += v[t] * a[jj][t];
         d = wave_sum(d) * tc;
@@ -79,43 +101,43 @@ struct RegPanel {
   // R on and above the diagonal, tau[c] in LDS.  vbuf: LDS scratch of 2*64*RPL doubles.
   // Requires m >= n.  Every thread of the work group must call it.  Column c = wo + NW*jc is owned
   // by wave wo in register slot jc; the slot loop is unrolled so every register index is static.
-  __device__ __forceinline__ void geqr2(int m, int n, double* vbuf, double* tau) {
+  __device__ __forceinline__ void geqr2(int m, int n, T* vbuf, T* tau) {
     const int w = wave_id(), l = lane_id();
 #pragma unroll
     for (int jc = 0; jc < CPW; ++jc) {
       for (int wo = 0; wo < NW; ++wo) {
         const int c = wo + NW * jc;
         if (c >= n) break;
-        double* vb = vbuf + (c & 1) * (64 * RPL);
+        T* vb = vbuf + (c & 1) * (64 * RPL);
         if (w == wo) {
-          double part = 0.0, alpha_l = 0.0;
+          T part = (T)0, alpha_l = (T)0;
 #pragma unroll
           for (int t = 0; t < RPL; ++t) {
             const int p = l + 64 * t;
-            const double x = a[jc][t];
+            const T x = a[jc][t];
             if (p > c && p < m) part += x * x;
             if (p == c) alpha_l = x;
           }
-          const double sigma = wave_sum(part);
-          const double alpha = wave_sum(alpha_l);  // exactly one lane holds it
-          double beta, tc, scal;
-          if (sigma == 0.0) {
-            tc = 0.0; beta = alpha; scal = 0.0;
+          const T sigma = wave_sum(part);
+          const T alpha = wave_sum(alpha_l);  // exactly one lane holds it
+          T beta, tc, scal;
+          if (sigma == (T)0) {
+            tc = (T)0; beta = alpha; scal = (T)0;
           } else {
-            const double nrm = sqrt(alpha * alpha + sigma);
-            beta = alpha >= 0.0 ? -nrm : nrm;
+            const T nrm = tsqrt(alpha * alpha + sigma);
+            beta = alpha >= (T)0 ? -nrm : nrm;
             tc = (beta - alpha) / beta;
-            scal = 1.0 / (alpha - beta);
+            scal = (T)1 / (alpha - beta);
           }
 #pragma unroll
           for (int t = 0; t < RPL; ++t) {
             const int p = l + 64 * t;
-            double v = 0.0;
-            if (p == c) v = 1.0;
+            T v = (T)0;
+            if (p == c) v = (T)1;
             else if (p > c && p < m) v = a[jc][t] * scal;
             vb[p] = v;
             if (p == c) a[jc][t] = beta;
-            else if (p > c) a[jc][t] = (p < m) ? v : 0.0;
+            else if (p > c) a[jc][t] = (p < m) ? v : (T)0;
           }
           if (l == 0) tau[c] = tc;
         }
@@ -127,7 +149,7 @@ struct RegPanel {
   }
 
   // Write the n x n upper triangle R (zeros below) to R (column-major, ldr).  Call after geqr2.
-  __device__ __forceinline__ void store_r(double* R, int ldr, int n) const {
+  __device__ __forceinline__ void store_r(T* R, int ldr, int n) const {
     const int w = wave_id(), l = lane_id();
 #pragma unroll
     for (int jj = 0; jj < CPW; ++jj) {
@@ -136,14 +158,14 @@ struct RegPanel {
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
           const int p = l + 64 * t;
-          if (p < n) R[p + j * ldr] = (p <= j) ? a[jj][t] : 0.0;
+          if (p < n) R[p + j * ldr] = (p <= j) ? a[jj][t] : (T)0;
         }
       }
     }
   }
 
   // Same, transposed: Rt[j + p*ldr] = R[p][j].
-  __device__ __forceinline__ void store_rt(double* Rt, int ldr, int n) const {
+  __device__ __forceinline__ void store_rt(T* Rt, int ldr, int n) const {
     const int w = wave_id(), l = lane_id();
 #pragma unroll
     for (int jj = 0; jj < CPW; ++jj) {
@@ -152,32 +174,32 @@ struct RegPanel {
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
           const int p = l + 64 * t;
-          if (p < n) Rt[j + p * ldr] = (p <= j) ? a[jj][t] : 0.0;
+          if (p < n) Rt[j + p * ldr] = (p <= j) ? a[jj][t] : (T)0;
         }
       }
     }
   }
 
   // Explicit Q (m x n) in place from the reflectors left by geqr2 (dorg2r), columns in reverse order.
-  __device__ __forceinline__ void org2r(int m, int n, double* vbuf, const double* tau) {
+  __device__ __forceinline__ void org2r(int m, int n, T* vbuf, const T* tau) {
     const int w = wave_id(), l = lane_id();
 #pragma unroll
     for (int jc = CPW - 1; jc >= 0; --jc) {
       for (int wo = NW - 1; wo >= 0; --wo) {
         const int c = wo + NW * jc;
         if (c >= n) continue;
-        double* vb = vbuf + (c & 1) * (64 * RPL);
-        const double tc = tau[c];
+        T* vb = vbuf + (c & 1) * (64 * RPL);
+        const T tc = tau[c];
         if (w == wo) {
 #pragma unroll
           for (int t = 0; t < RPL; ++t) {
             const int p = l + 64 * t;
-            double v = 0.0;
-            if (p == c) v = 1.0;
+            T v = (T)0;
+            if (p == c) v = (T)1;
             else if (p > c && p < m) v = a[jc][t];
             vb[p] = v;
             // column c of Q = H_c e_c = e_c - tau v
-            a[jc][t] = (p == c) ? 1.0 - tc : ((p > c && p < m) ? -tc * v : 0.0);
+            a[jc][t] = (p == c) ? (T)1 - tc : ((p > c && p < m) ? -tc * v : (T)0);
           }
         }
         __syncthreads();
@@ -191,30 +213,31 @@ struct RegPanel {
 // C[a + j*ldc] = sc[a] * (A[:, a]^T P[:, j]) for a < na, j < n; A global (m x na, column-major, lda),
 // sc optional column scale.  Each wave produces the entries of its own panel columns.  The columns of A
 // are fetched AU at a time so that several loads are in flight per wave.
-template <int RPL, int NW, int CPW>
-__device__ __forceinline__ void panel_coef(const RegPanel<RPL, NW, CPW>& P, const double* A, int64_t lda, int m,
-                                           int na, int n, const double* sc, double* C, int ldc) {
+// BF: the entries of A are rounded to bfloat16 (T = float).
+template <bool BF = false, int RPL, int NW, int CPW, class T, class S>
+__device__ __forceinline__ void panel_coef(const RegPanel<RPL, NW, CPW, T>& P, const S* A, int64_t lda, int m,
+                                           int na, int n, const T* sc, T* C, int ldc) {
   constexpr int AU = CPW <= 4 ? 2 : 1;
   const int w = wave_id(), l = lane_id();
   for (int a0 = 0; a0 < na; a0 += AU) {
-    double av[AU][RPL];
+    T av[AU][RPL];
 #pragma unroll
     for (int u = 0; u < AU; ++u)
 #pragma unroll
       for (int t = 0; t < RPL; ++t) {
         const int p = l + 64 * t;
-        av[u][t] = (a0 + u < na && p < m) ? A[p + (int64_t)(a0 + u) * lda] : 0.0;
+        av[u][t] = (a0 + u < na && p < m) ? cvt<T, BF>(A[p + (int64_t)(a0 + u) * lda]) : (T)0;
       }
 #pragma unroll
     for (int u = 0; u < AU; ++u) {
       const int a = a0 + u;
       if (a < na) {
-        const double s = sc ? sc[a] : 1.0;
+        const T s = sc ? sc[a] : (T)1;
 #pragma unroll
         for (int jj = 0; jj < CPW; ++jj) {
-          const int j = RegPanel<RPL, NW, CPW>::col_of(w, jj);
+          const int j = RegPanel<RPL, NW, CPW, T>::col_of(w, jj);
           if (j < n) {
-            double d = 0.0;
+            T d = (T)0;
 #pragma unroll
             for (int t = 0; t < RPL; ++t) d += av[u][t] * P.a[jj][t];
             d = wave_sum(d);
@@ -228,30 +251,31 @@ __device__ __forceinline__ void panel_coef(const RegPanel<RPL, NW, CPW>& P, cons
 
 // P[:, j] += alpha * sum_a A[:, a] sc[a] C[a + j*ldc] for j < n (C must be visible to every wave).
 // Columns of A fetched AU at a time; the accumulation order over a is sequential.
-template <int RPL, int NW, int CPW>
-__device__ __forceinline__ void panel_axpy(RegPanel<RPL, NW, CPW>& P, const double* A, int64_t lda, int m, int na,
-                                           int n, const double* sc, const double* C, int ldc, double alpha) {
+// BF: the entries of A and of C are rounded to bfloat16 (T = float; alpha and sc must be exact in it).
+template <bool BF = false, int RPL, int NW, int CPW, class T, class S>
+__device__ __forceinline__ void panel_axpy(RegPanel<RPL, NW, CPW, T>& P, const S* A, int64_t lda, int m, int na,
+                                           int n, const T* sc, const T* C, int ldc, T alpha) {
   constexpr int AU = CPW <= 4 ? 2 : 1;
   const int w = wave_id(), l = lane_id();
   for (int a0 = 0; a0 < na; a0 += AU) {
-    double av[AU][RPL];
+    T av[AU][RPL];
 #pragma unroll
     for (int u = 0; u < AU; ++u)
 #pragma unroll
       for (int t = 0; t < RPL; ++t) {
         const int p = l + 64 * t;
-        av[u][t] = (a0 + u < na && p < m) ? A[p + (int64_t)(a0 + u) * lda] : 0.0;
+        av[u][t] = (a0 + u < na && p < m) ? cvt<T, BF>(A[p + (int64_t)(a0 + u) * lda]) : (T)0;
       }
 #pragma unroll
     for (int u = 0; u < AU; ++u) {
       const int a = a0 + u;
       if (a < na) {
-        const double s = alpha * (sc ? sc[a] : 1.0);
+        const T s = alpha * (sc ? sc[a] : (T)1);
 #pragma unroll
         for (int jj = 0; jj < CPW; ++jj) {
-          const int j = RegPanel<RPL, NW, CPW>::col_of(w, jj);
+          const int j = RegPanel<RPL, NW, CPW, T>::col_of(w, jj);
           if (j < n) {
-            const double f = s * C[a + j * ldc];
+            const T f = s * cvt<T, BF>(C[a + j * ldc]);
 #pragma unroll
             for (int t = 0; t < RPL; ++t) P.a[jj][t] = fma(f, av[u][t], P.a[jj][t]);
           }
@@ -274,16 +298,20 @@ __device__ __forceinline__ void wave_lds_sync() {
 // sqrt(al be)) or either column is below the noise floor, else the cosine and sine of the rotation that
 // makes the columns orthogonal: t = tan(angle) = sign(d) h / (|d| + sqrt(d^2 + h^2)) with d = be - al and
 // h = 2 ga (t = 1 when d = 0), cs = 1 / sqrt(1 + t^2), sn = cs t.
-__device__ __forceinline__ bool jacobi_rotation(double al, double be, double ga, double floor2, double tol,
-                                                double* cs, double* sn) {
-  if (!(ga != 0.0 && al > floor2 && be > floor2 && ga * ga > tol * tol * al * be)) return false;
-  const double d = be - al, h = 2.0 * ga;
-  const double t = d == 0.0 ? 1.0 : (d > 0.0 ? h : -h) / (fabs(d) + sqrt(fma(d, d, h * h)));
-  const double c = 1.0 / sqrt(fma(t, t, 1.0));
+template <class T>
+__device__ __forceinline__ bool jacobi_rotation(T al, T be, T ga, T floor2, T tol, T* cs, T* sn) {
+  if (!(ga != (T)0 && al > floor2 && be > floor2 && ga * ga > tol * tol * al * be)) return false;
+  const T d = be - al, h = (T)2 * ga;
+  const T t = d == (T)0 ? (T)1 : (d > (T)0 ? h : -h) / (fabs(d) + tsqrt(fma(d, d, h * h)));
+  const T c = (T)1 / tsqrt(fma(t, t, (T)1));
   *cs = c;
   *sn = c * t;
   return true;
 }
+
+// Largest finite value of T (column norms beyond it are reported as non-finite).
+__device__ __forceinline__ constexpr double huge_of(double) { return 1.7976931348623157e308; }
+__device__ __forceinline__ constexpr float huge_of(float) { return 3.4028234663852886e38f; }
 
 // jacobi_svd for n <= 128 / TP (TP = 2, 4, 8 or 16 lanes per column pair), run by one wave alone: its 64
 // lanes hold every column pair of a round (rows tl, tl + TP, ... of both columns and of J in registers), so
@@ -291,23 +319,22 @@ __device__ __forceinline__ bool jacobi_rotation(double al, double be, double ga,
 // round, and the group sums are DPP exchanges.  Same ordering, rotation, noise floor and stopping rule as
 // jacobi_svd.  G, J, sig and perm must be in LDS (the wave-level fence orders LDS accesses only).  Only one
 // wave may call it; the caller synchronises the work group after.
-template <int TP>
-__device__ int jacobi_svd_w0(double* G, int ldg, double* J, int ldj, int n, double* sig, int* perm,
-                             int max_sweeps) {
+template <int TP, class T>
+__device__ int jacobi_svd_w0(T* G, int ldg, T* J, int ldj, int n, T* sig, int* perm, int max_sweeps) {
   constexpr int UR = 128 / (TP * TP) > 0 ? 128 / (TP * TP) : 1;   // rows per lane
   constexpr bool HOIST = UR <= 8;   // J's rows loaded with G's (registers permitting)
   const int l = lane_id();
   for (int e = l; e < n * n; e += 64) {
     const int r = e % n, c = e / n;
-    J[r + c * ldj] = (r == c) ? 1.0 : 0.0;
+    J[r + c * ldj] = (r == c) ? (T)1 : (T)0;
   }
-  double cn = 0.0;
+  T cn = (T)0;
   if (l < n)
-    for (int i = 0; i < n; ++i) { const double g = G[i + l * ldg]; cn = fma(g, g, cn); }
-  const double floor2 = 4.930380657631324e-32 * wave_sum(cn);   // eps^2 ||G||_F^2
+    for (int i = 0; i < n; ++i) { const T g = G[i + l * ldg]; cn = fma(g, g, cn); }
+  const T floor2 = Prec<T>::tiny2 * wave_sum(cn);   // eps^2 ||G||_F^2
   const int np = (n + 1) & ~1, npairs = np / 2, m = np - 1;
   const int pi = l / TP, tl = l % TP;
-  const double tol = 2.220446049250313e-16 * sqrt((double)n);
+  const T tol = Prec<T>::eps * tsqrt((T)n);
   int sweeps = -1;
   wave_lds_sync();
   for (int s = 0; s < max_sweeps && n > 1; ++s) {
@@ -323,24 +350,24 @@ __device__ int jacobi_svd_w0(double* G, int ldg, double* J, int ldj, int n, doub
       pp = pp + 1 == m ? 0 : pp + 1;
       qq = qq + 1 == m ? 0 : qq + 1;
       if (q < n) {
-        double gp[UR], gq[UR], jp[HOIST ? UR : 1], jq[HOIST ? UR : 1];
+        T gp[UR], gq[UR], jp[HOIST ? UR : 1], jq[HOIST ? UR : 1];
 #pragma unroll
         for (int u = 0; u < UR; ++u) {
           const int i = tl + TP * u;
-          gp[u] = i < n ? G[i + p * ldg] : 0.0;
-          gq[u] = i < n ? G[i + q * ldg] : 0.0;
+          gp[u] = i < n ? G[i + p * ldg] : (T)0;
+          gq[u] = i < n ? G[i + q * ldg] : (T)0;
           if (HOIST) {
-            jp[HOIST ? u : 0] = i < n ? J[i + p * ldj] : 0.0;
-            jq[HOIST ? u : 0] = i < n ? J[i + q * ldj] : 0.0;
+            jp[HOIST ? u : 0] = i < n ? J[i + p * ldj] : (T)0;
+            jq[HOIST ? u : 0] = i < n ? J[i + q * ldj] : (T)0;
           }
         }
-        double al = 0.0, be = 0.0, ga = 0.0;
+        T al = (T)0, be = (T)0, ga = (T)0;
 #pragma unroll
         for (int u = 0; u < UR; ++u) {
           al = fma(gp[u], gp[u], al); be = fma(gq[u], gq[u], be); ga = fma(gp[u], gq[u], ga);
         }
         al = group_sum_dpp<TP>(al); be = group_sum_dpp<TP>(be); ga = group_sum_dpp<TP>(ga);
-        double cs, sn;
+        T cs, sn;
         if (jacobi_rotation(al, be, ga, floor2, tol, &cs, &sn)) {
 #pragma unroll
           for (int u = 0; u < UR; ++u) {
@@ -348,8 +375,8 @@ __device__ int jacobi_svd_w0(double* G, int ldg, double* J, int ldj, int n, doub
             if (i < n) {
               G[i + p * ldg] = cs * gp[u] - sn * gq[u];
               G[i + q * ldg] = sn * gp[u] + cs * gq[u];
-              const double a = HOIST ? jp[HOIST ? u : 0] : J[i + p * ldj];
-              const double c = HOIST ? jq[HOIST ? u : 0] : J[i + q * ldj];
+              const T a = HOIST ? jp[HOIST ? u : 0] : J[i + p * ldj];
+              const T c = HOIST ? jq[HOIST ? u : 0] : J[i + q * ldj];
               J[i + p * ldj] = cs * a - sn * c;
               J[i + q * ldj] = sn * a + cs * c;
             }
@@ -363,22 +390,22 @@ __device__ int jacobi_svd_w0(double* G, int ldg, double* J, int ldj, int n, doub
   }
   if (n <= 1) sweeps = 0;
   if (l < n) {
-    double acc = 0.0;
-    for (int i = 0; i < n; ++i) { const double g = G[i + l * ldg]; acc += g * g; }
-    const double sv = sqrt(acc);
-    sig[l] = (sv == sv && sv <= 1.7976931348623157e308) ? sv : -1.0;
+    T acc = (T)0;
+    for (int i = 0; i < n; ++i) { const T g = G[i + l * ldg]; acc += g * g; }
+    const T sv = tsqrt(acc);
+    sig[l] = (sv == sv && sv <= huge_of(sv)) ? sv : (T)-1;
   }
   wave_lds_sync();
   bool bad = false;
   if (l < n) {
-    const double sj = sig[l];
+    const T sj = sig[l];
     int rank = 0;
     for (int i = 0; i < n; ++i) {
-      const double si = sig[i];
+      const T si = sig[i];
       rank += (si > sj) || (si == sj && i < l);
     }
     perm[rank] = l;
-    bad = sj < 0.0;
+    bad = sj < (T)0;
   }
   wave_lds_sync();
   return __any(bad) ? -2 : sweeps;
@@ -390,35 +417,34 @@ __device__ int jacobi_svd_w0(double* G, int ldg, double* J, int ldj, int n, doub
 // index).  TP threads cooperate on one column pair.  Returns the number of sweeps (-1: no convergence
 // within max_sweeps, -2: a non-finite column norm).  G, J, sig, perm may live in LDS or global memory; flag is an LDS int.
 // sig must hold n entries (it also carries the column norms used for the noise floor).
-template <int TP>
-__device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double* sig, int* perm, int* flag,
-                          int max_sweeps) {
+template <int TP, class T>
+__device__ int jacobi_svd(T* G, int ldg, T* J, int ldj, int n, T* sig, int* perm, int* flag, int max_sweeps) {
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int e = tid; e < n * n; e += nt) {
     const int r = e % n, c = e / n;
-    J[r + c * ldj] = (r == c) ? 1.0 : 0.0;
+    J[r + c * ldj] = (r == c) ? (T)1 : (T)0;
   }
   const int np = (n + 1) & ~1;      // players, padded to even
   const int npairs = np / 2;
   const int pi = tid / TP, tl = tid % TP;
-  const double tol = 2.220446049250313e-16 * sqrt((double)n);
+  const T tol = Prec<T>::eps * tsqrt((T)n);
   // Columns whose squared norm is below (eps ||G||_F)^2 are rounding noise relative to the matrix: they
   // are not rotated (their singular values stay below eps ||G||_F either way), which keeps noise-level
   // columns from being stirred sweep after sweep by the rotations of the large ones.
-  __shared__ double jac_floor;
+  __shared__ T jac_floor;
   for (int c = tid; c < n; c += nt) {
-    double acc = 0.0;
-    for (int i = 0; i < n; ++i) { const double g = G[i + c * ldg]; acc = fma(g, g, acc); }
+    T acc = (T)0;
+    for (int i = 0; i < n; ++i) { const T g = G[i + c * ldg]; acc = fma(g, g, acc); }
     sig[c] = acc;
   }
   __syncthreads();
   if (tid == 0) {
-    double fro2 = 0.0;
+    T fro2 = (T)0;
     for (int c = 0; c < n; ++c) fro2 += sig[c];
-    jac_floor = 4.930380657631324e-32 * fro2;   // eps^2 ||G||_F^2
+    jac_floor = Prec<T>::tiny2 * fro2;   // eps^2 ||G||_F^2
   }
   __syncthreads();
-  const double floor2 = jac_floor;
+  const T floor2 = jac_floor;
   int sweeps = -1;
   for (int s = 0; s < max_sweeps && n > 1; ++s) {
     if (tid == 0) *flag = 0;
@@ -430,19 +456,19 @@ __device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double*
         else { p = (r + pi) % (np - 1); q = (r - pi + (np - 1)) % (np - 1); }
         if (p > q) { const int x = p; p = q; q = x; }
         if (q < n) {
-          double al = 0.0, be = 0.0, ga = 0.0;
+          T al = (T)0, be = (T)0, ga = (T)0;
           for (int i = tl; i < n; i += TP) {
-            const double gp = G[i + p * ldg], gq = G[i + q * ldg];
+            const T gp = G[i + p * ldg], gq = G[i + q * ldg];
             al += gp * gp; be += gq * gq; ga += gp * gq;
           }
           al = group_sum_dpp<TP>(al); be = group_sum_dpp<TP>(be); ga = group_sum_dpp<TP>(ga);
-          double cs, sn;
+          T cs, sn;
           if (jacobi_rotation(al, be, ga, floor2, tol, &cs, &sn)) {
             for (int i = tl; i < n; i += TP) {
-              const double gp = G[i + p * ldg], gq = G[i + q * ldg];
+              const T gp = G[i + p * ldg], gq = G[i + q * ldg];
               G[i + p * ldg] = cs * gp - sn * gq;
               G[i + q * ldg] = sn * gp + cs * gq;
-              const double jp = J[i + p * ldj], jq = J[i + q * ldj];
+              const T jp = J[i + p * ldj], jq = J[i + q * ldj];
               J[i + p * ldj] = cs * jp - sn * jq;
               J[i + q * ldj] = sn * jp + cs * jq;
             }
@@ -458,26 +484,26 @@ __device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double*
   }
   if (n <= 1) sweeps = 0;
   for (int j = tid; j < n; j += nt) {
-    double acc = 0.0;
-    for (int i = 0; i < n; ++i) { const double g = G[i + j * ldg]; acc += g * g; }
-    const double sv = sqrt(acc);
+    T acc = (T)0;
+    for (int i = 0; i < n; ++i) { const T g = G[i + j * ldg]; acc += g * g; }
+    const T sv = tsqrt(acc);
     // a non-finite column norm is ranked last (as -1) so perm stays a permutation; the caller sees it
     // through the returned status
-    sig[j] = (sv == sv && sv <= 1.7976931348623157e308) ? sv : -1.0;
+    sig[j] = (sv == sv && sv <= huge_of(sv)) ? sv : (T)-1;
   }
   __syncthreads();
   for (int j = tid; j < n; j += nt) {
-    const double sj = sig[j];
+    const T sj = sig[j];
     int rank = 0;
     for (int i = 0; i < n; ++i) {
-      const double si = sig[i];
+      const T si = sig[i];
       rank += (si > sj) || (si == sj && i < j);
     }
     perm[rank] = j;
   }
   __syncthreads();
   for (int j = 0; j < n; ++j)
-    if (sig[j] < 0.0) return -2;
+    if (sig[j] < (T)0) return -2;
   return sweeps;
 }
 
@@ -485,32 +511,31 @@ __device__ int jacobi_svd(double* G, int ldg, double* J, int ldj, int n, double*
 // pair): each pair's rows tl, tl + TP, ... of both columns (and of J when UR <= 8) are read once per round
 // into registers, the group sums are DPP exchanges, and the work group synchronises once per round.  Same
 // ordering, rotation, noise floor and stopping rule as jacobi_svd.  Every thread of the work group calls it.
-template <int TP, int UR>
-__device__ int jacobi_svd_mw(double* G, int ldg, double* J, int ldj, int n, double* sig, int* perm,
-                             int max_sweeps) {
+template <int TP, int UR, class T>
+__device__ int jacobi_svd_mw(T* G, int ldg, T* J, int ldj, int n, T* sig, int* perm, int max_sweeps) {
   constexpr bool HOIST = UR <= 8;
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int e = tid; e < n * n; e += nt) {
     const int r = e % n, c = e / n;
-    J[r + c * ldj] = (r == c) ? 1.0 : 0.0;
+    J[r + c * ldj] = (r == c) ? (T)1 : (T)0;
   }
-  __shared__ double jac_floor_mw;
+  __shared__ T jac_floor_mw;
   for (int c = tid; c < n; c += nt) {
-    double acc = 0.0;
-    for (int i = 0; i < n; ++i) { const double g = G[i + c * ldg]; acc = fma(g, g, acc); }
+    T acc = (T)0;
+    for (int i = 0; i < n; ++i) { const T g = G[i + c * ldg]; acc = fma(g, g, acc); }
     sig[c] = acc;
   }
   __syncthreads();
   if (tid == 0) {
-    double fro2 = 0.0;
+    T fro2 = (T)0;
     for (int c = 0; c < n; ++c) fro2 += sig[c];
-    jac_floor_mw = 4.930380657631324e-32 * fro2;   // eps^2 ||G||_F^2
+    jac_floor_mw = Prec<T>::tiny2 * fro2;   // eps^2 ||G||_F^2
   }
   __syncthreads();
-  const double floor2 = jac_floor_mw;
+  const T floor2 = jac_floor_mw;
   const int np = (n + 1) & ~1, npairs = np / 2, m = np - 1;
   const int pi = tid / TP, tl = tid % TP;
-  const double tol = 2.220446049250313e-16 * sqrt((double)n);
+  const T tol = Prec<T>::eps * tsqrt((T)n);
   int sweeps = -1;
   for (int s = 0; s < max_sweeps && n > 1; ++s) {
     int rotated = 0;
@@ -524,24 +549,24 @@ __device__ int jacobi_svd_mw(double* G, int ldg, double* J, int ldj, int n, doub
       pp = pp + 1 == m ? 0 : pp + 1;
       qq = qq + 1 == m ? 0 : qq + 1;
       if (q < n) {
-        double gp[UR], gq[UR], jp[HOIST ? UR : 1], jq[HOIST ? UR : 1];
+        T gp[UR], gq[UR], jp[HOIST ? UR : 1], jq[HOIST ? UR : 1];
 #pragma unroll
         for (int u = 0; u < UR; ++u) {
           const int i = tl + TP * u;
-          gp[u] = i < n ? G[i + p * ldg] : 0.0;
-          gq[u] = i < n ? G[i + q * ldg] : 0.0;
+          gp[u] = i < n ? G[i + p * ldg] : (T)0;
+          gq[u] = i < n ? G[i + q * ldg] : (T)0;
           if (HOIST) {
-            jp[HOIST ? u : 0] = i < n ? J[i + p * ldj] : 0.0;
-            jq[HOIST ? u : 0] = i < n ? J[i + q * ldj] : 0.0;
+            jp[HOIST ? u : 0] = i < n ? J[i + p * ldj] : (T)0;
+            jq[HOIST ? u : 0] = i < n ? J[i + q * ldj] : (T)0;
           }
         }
-        double al = 0.0, be = 0.0, ga = 0.0;
+        T al = (T)0, be = (T)0, ga = (T)0;
 #pragma unroll
         for (int u = 0; u < UR; ++u) {
           al = fma(gp[u], gp[u], al); be = fma(gq[u], gq[u], be); ga = fma(gp[u], gq[u], ga);
         }
         al = group_sum_dpp<TP>(al); be = group_sum_dpp<TP>(be); ga = group_sum_dpp<TP>(ga);
-        double cs, sn;
+        T cs, sn;
         if (jacobi_rotation(al, be, ga, floor2, tol, &cs, &sn)) {
 #pragma unroll
           for (int u = 0; u < UR; ++u) {
@@ -549,8 +574,8 @@ __device__ int jacobi_svd_mw(double* G, int ldg, double* J, int ldj, int n, doub
             if (i < n) {
               G[i + p * ldg] = cs * gp[u] - sn * gq[u];
               G[i + q * ldg] = sn * gp[u] + cs * gq[u];
-              const double a = HOIST ? jp[HOIST ? u : 0] : J[i + p * ldj];
-              const double c = HOIST ? jq[HOIST ? u : 0] : J[i + q * ldj];
+              const T a = HOIST ? jp[HOIST ? u : 0] : J[i + p * ldj];
+              const T c = HOIST ? jq[HOIST ? u : 0] : J[i + q * ldj];
               J[i + p * ldj] = cs * a - sn * c;
               J[i + q * ldj] = sn * a + cs * c;
             }
@@ -565,24 +590,24 @@ __device__ int jacobi_svd_mw(double* G, int ldg, double* J, int ldj, int n, doub
   }
   if (n <= 1) sweeps = 0;
   for (int j = tid; j < n; j += nt) {
-    double acc = 0.0;
-    for (int i = 0; i < n; ++i) { const double g = G[i + j * ldg]; acc += g * g; }
-    const double sv = sqrt(acc);
-    sig[j] = (sv == sv && sv <= 1.7976931348623157e308) ? sv : -1.0;
+    T acc = (T)0;
+    for (int i = 0; i < n; ++i) { const T g = G[i + j * ldg]; acc += g * g; }
+    const T sv = tsqrt(acc);
+    sig[j] = (sv == sv && sv <= huge_of(sv)) ? sv : (T)-1;
   }
   __syncthreads();
   for (int j = tid; j < n; j += nt) {
-    const double sj = sig[j];
+    const T sj = sig[j];
     int rank = 0;
     for (int i = 0; i < n; ++i) {
-      const double si = sig[i];
+      const T si = sig[i];
       rank += (si > sj) || (si == sj && i < j);
     }
     perm[rank] = j;
   }
   __syncthreads();   // a full barrier with its memory fence: callers read sig and perm next
   for (int j = 0; j < n; ++j)
-    if (sig[j] < 0.0) return -2;
+    if (sig[j] < (T)0) return -2;
   return sweeps;
 }
 
@@ -590,9 +615,8 @@ __device__ int jacobi_svd_mw(double* G, int ldg, double* J, int ldj, int n, doub
 // n <= 32, in which case G, J, sig and perm must be in LDS; with 512 threads, 16 lanes per pair up to
 // n = 64 and 8 lanes per pair up to n = 128 (jacobi_svd_mw).  Every thread of the work group calls it; flag
 // is an LDS int (it also carries the one-wave result to the other waves).
-template <int TP>
-__device__ int jacobi_svd_auto(double* G, int ldg, double* J, int ldj, int n, double* sig, int* perm, int* flag,
-                               int max_sweeps) {
+template <int TP, class T>
+__device__ int jacobi_svd_auto(T* G, int ldg, T* J, int ldj, int n, T* sig, int* perm, int* flag, int max_sweeps) {
   if (n <= 32) {
     if (wave_id() == 0) {
       int r;

@@ -59,6 +59,22 @@ struct TimedLaunch {
 
 }  // namespace
 
+// What the factor resident in a slot is: the evaluation at theta (rule, value, precision) with its result,
+// or nothing (before the first evaluation, after an error, or after a frozen-rank replay, whose factor is not
+// the plain one at its theta).
+struct SlotState {
+  bool valid = false;
+  double theta[3] = {0.0, 0.0, 0.0};
+  int rule = -1, precision = -1;
+  double value = 0.0;
+  double out3[3] = {0.0, 0.0, 0.0};
+  int status = TLR_OK, failed_at = -1;
+  bool matches(const double th[3], int r, double v, int p) const {
+    return valid && th[0] == theta[0] && th[1] == theta[1] && th[2] == theta[2] && r == rule && v == value &&
+           p == precision;
+  }
+};
+
 struct tlr_ctx {
   int64_t n = 0;
   int b = 0, nB = 0, npan = 0, max_batch = 0;
@@ -73,6 +89,8 @@ struct tlr_ctx {
   int nset = 0;                        // workspace sets allocated: min(NCS, max_batch)
   std::vector<void*> allocs;
   std::vector<Slot> slots;
+  std::vector<SlotState> sstate;       // per slot: the factor it holds
+  int64_t factorizations = 0;          // evaluations run (theta factorized), for reports
   SlotPtrs* d_slots = nullptr;
   int64_t scr_stride = 0;
   double* scr_pool = nullptr;          // recompression scratch, one scr_stride per resident work group
@@ -88,7 +106,23 @@ struct tlr_ctx {
   int64_t wlp_cap = 0;
   double* scr_pan[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};   // scratch of the panel launches
   double* scr_help[2] = {nullptr, nullptr};   // scratch of the second 17-32 launches (panel, rest)
-  int recomp_grid[RCLASSES] = {0, 0, 0, 0};   // persistent grid of each recompression class
+  // arena mode (the allocation-free entry point): every device buffer is carved from [arena, arena +
+  // arena_cap), one stream (the null stream), no events, no pinned staging, persistent grids of at most
+  // grid_cap work groups
+  char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  bool arena_short = false;
+  int grid_cap = 0;
+  bool single_stream = false;
+  std::vector<double> h_thr_host;
+  std::vector<int> h_replay_host;
+  int recomp_grid[3][RCLASSES] = {};   // persistent grid of each recompression class, per arithmetic (PM_*)
+  int recomp_grid_max[3][RCLASSES] = {};   // resident capacity (the grids above never exceed it)
+  double guard = GUARD_DEFAULT;        // tunables (tlr_ctx_set_tunable)
+  int diag_wg = 9;
+  int2* wlr = nullptr;                 // redo lists (panel x level) and their counters
+  int* wlr_count = nullptr;
+  int* rep_rpl = nullptr;              // max_batch records of recompression arithmetic (nlog each)
   int* d_retry = nullptr;              // per slot: tiles redone with the wider sketch
   double* d_thr = nullptr;             // per slot (thr_rel, thr_abs) of the current batch
   double* h_thr = nullptr;             // pinned staging of d_thr
@@ -116,8 +150,8 @@ struct tlr_ctx {
   ~tlr_ctx() {
     if (pred_buf) (void)hipFree(pred_buf);
     if (stash) (void)hipFree(stash);
-    if (h_thr) (void)hipHostFree(h_thr);
-    if (h_replay) (void)hipHostFree(h_replay);
+    if (h_thr && !arena) (void)hipHostFree(h_thr);
+    if (h_replay && !arena) (void)hipHostFree(h_replay);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
     if (ev_list) (void)hipEventDestroy(ev_list);
@@ -148,6 +182,18 @@ struct tlr_ctx {
   }
   template <class T>
   hipError_t alloc(T** p, size_t count) {
+    if (arena) {   // carve, 256-byte aligned
+      const size_t bytes = count * sizeof(T) > 0 ? count * sizeof(T) : 16;
+      const size_t off = (arena_used + 255) & ~(size_t)255;
+      if (off + bytes > arena_cap) {
+        *p = nullptr;
+        arena_short = true;
+        return hipErrorOutOfMemory;
+      }
+      *p = (T*)(arena + off);
+      arena_used = off + bytes;
+      return hipSuccess;
+    }
     void* q = nullptr;
     hipError_t e = hipMalloc(&q, count * sizeof(T) > 0 ? count * sizeof(T) : 16);
     if (e == hipSuccess) allocs.push_back(q);
@@ -166,6 +212,13 @@ struct tlr_ctx {
     a.wlp = wlp;
     a.wlp_count = wlp_count;
     a.wlp_cap = wlp_cap;
+    a.wlr = wlr;
+    a.wlr_count = wlr_count;
+    a.wlr_cap = wl_cap;
+    a.prec_mode = PM_FP64;
+    a.guard = guard;
+    a.diag_wg = diag_wg;
+    a.rep_rpl = rep_rpl;
     a.rule_fixed = 0;
     a.mixed = 0;
     a.thr = d_thr;
@@ -212,13 +265,15 @@ int tlr_device_info(char* buf, int32_t len) {
   return TLR_OK;
 }
 
-int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, int32_t b, int32_t max_batch) {
-  if (!out || !S || !y || n < 2 || b < 2 || max_batch < 1) return set_err(TLR_ERR_ARG, "invalid argument");
+}  // extern "C"
+
+// Fill a fresh context (its arena fields set for arena mode); on failure the caller destroys it.
+static int ctx_build(tlr_ctx* c, const double* S, const double* y, int64_t n, int32_t b, int32_t max_batch) {
+  if (!S || !y || n < 2 || b < 2 || max_batch < 1) return set_err(TLR_ERR_ARG, "invalid argument");
   if (b > 512) return set_err(TLR_ERR_ARG, "tile size b > 512 is not supported");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(TLR_ERR_NODEVICE, "no HIP device");
-  tlr_ctx* c = new tlr_ctx();
-  auto fail = [&](int code) { delete c; return code; };
+  auto fail = [&](int code) { return code; };
   c->n = n; c->b = b;
   c->nB = (int)((n + b - 1) / b);
   c->npan = (b + 31) / 32;
@@ -227,14 +282,20 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
   const int64_t m = c->nB - 1;
   c->npairs_max = m * (m - 1) / 2;
   c->kcap = K2 < b ? K2 : b;
-  if (c->ntile > 65535) return fail(set_err(TLR_ERR_ARG, "more than 65535 off-diagonal tiles"));
+  // pair indices of one step and the tile count stay within int range up to nB = 46,341
+  if (c->nB > 46340) return fail(set_err(TLR_ERR_ARG, "more than 46340 tiles per side"));
   hipError_t e;
 #define ACHK(expr)                                                                           \
   do {                                                                                       \
     e = (expr);                                                                              \
     if (e != hipSuccess) return fail(set_err(TLR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e))); \
   } while (0)
-  ACHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if (c->arena) {
+    c->stream = nullptr;   // the null stream
+    c->single_stream = true;
+  } else {
+    ACHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  }
   // sites as structure of arrays
   std::vector<double> hx(n), hy(n);
   for (int64_t i = 0; i < n; ++i) { hx[i] = S[2 * i]; hy[i] = S[2 * i + 1]; }
@@ -268,33 +329,43 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
     int dev = 0, cus = 0, per_cu[RCLASSES];
     ACHK(hipGetDevice(&dev));
     ACHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    ACHK(recompress_occupancy(b, per_cu));
-    int64_t gsum = 0;
-    for (int q = 0; q < RCLASSES; ++q) {
-      c->recomp_grid[q] = (per_cu[q] > 0 ? per_cu[q] : 1) * (cus > 0 ? cus : 1);
-      gsum += c->recomp_grid[q];
+    int gmax[RCLASSES] = {0, 0, 0, 0};   // the largest grid of each class over the arithmetic modes
+    for (int pm = 0; pm < 3; ++pm) {
+      ACHK(recompress_occupancy(b, pm, per_cu));
+      for (int q = 0; q < RCLASSES; ++q) {
+        c->recomp_grid[pm][q] = (per_cu[q] > 0 ? per_cu[q] : 1) * (cus > 0 ? cus : 1);
+        if (c->grid_cap > 0 && c->recomp_grid[pm][q] > c->grid_cap) c->recomp_grid[pm][q] = c->grid_cap;
+        c->recomp_grid_max[pm][q] = c->recomp_grid[pm][q];
+        gmax[q] = c->recomp_grid[pm][q] > gmax[q] ? c->recomp_grid[pm][q] : gmax[q];
+      }
     }
+    int64_t gsum = 0;
+    for (int q = 0; q < RCLASSES; ++q) gsum += gmax[q];
     // the classes of one step run concurrently (one stream each) and so do the panel launches, so each
     // class has its own scratch twice over, and the 17-32 class, launched on two streams, four times
-    ACHK(c->alloc(&c->scr_pool, (size_t)(2 * gsum + 2 * c->recomp_grid[2]) * c->scr_stride));
+    ACHK(c->alloc(&c->scr_pool, (size_t)(2 * gsum + 2 * gmax[2]) * c->scr_stride));
     int64_t off = 0;
     for (int q = 0; q < RCLASSES; ++q) {
       c->scr_cls[q] = c->scr_pool + off * c->scr_stride;
-      off += c->recomp_grid[q];
+      off += gmax[q];
       c->scr_pan[q] = c->scr_pool + off * c->scr_stride;
-      off += c->recomp_grid[q];
+      off += gmax[q];
     }
     for (int h = 0; h < 2; ++h) {
       c->scr_help[h] = c->scr_pool + off * c->scr_stride;
-      off += c->recomp_grid[2];
+      off += gmax[2];
     }
     c->cls_stream[0] = c->stream;
-    for (int q = 1; q <= 2; ++q) {
-      ACHK(hipStreamCreateWithFlags(&c->cls_stream[q], hipStreamNonBlocking));
-      ACHK(hipEventCreateWithFlags(&c->ev_cls[q], hipEventDisableTiming));
-      ACHK(hipEventCreateWithFlags(&c->ev_pan[q], hipEventDisableTiming));
+    if (c->single_stream) {
+      c->cls_stream[1] = c->cls_stream[2] = c->stream;
+    } else {
+      for (int q = 1; q <= 2; ++q) {
+        ACHK(hipStreamCreateWithFlags(&c->cls_stream[q], hipStreamNonBlocking));
+        ACHK(hipEventCreateWithFlags(&c->ev_cls[q], hipEventDisableTiming));
+        ACHK(hipEventCreateWithFlags(&c->ev_pan[q], hipEventDisableTiming));
+      }
+      ACHK(hipEventCreateWithFlags(&c->ev_list, hipEventDisableTiming));
     }
-    ACHK(hipEventCreateWithFlags(&c->ev_list, hipEventDisableTiming));
     const int64_t npm = c->npairs_max > 0 ? c->npairs_max : 1;
     c->wl_cap = npm * max_batch;
     ACHK(c->alloc(&c->wl, (size_t)RCLASSES * c->wl_cap));
@@ -302,8 +373,12 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
     c->wlp_cap = (int64_t)(c->nB > 0 ? c->nB : 1) * max_batch;
     ACHK(c->alloc(&c->wlp, (size_t)RCLASSES * c->wlp_cap));
     ACHK(c->alloc(&c->wlp_count, (size_t)WL_STRIDE * RCLASSES * c->nB));
+    // redo lists of the reduced-precision modes: (panel, level) lists of wl_cap entries, counters per step
+    ACHK(c->alloc(&c->wlr, (size_t)4 * c->wl_cap));
+    ACHK(c->alloc(&c->wlr_count, (size_t)WL_STRIDE * 4 * c->nB));
   }
   c->slots.resize(max_batch);
+  c->sstate.assign(max_batch, SlotState());
   for (int s = 0; s < max_batch; ++s) {
     SlotPtrs& p = c->slots[s].p;
     ACHK(c->alloc(&p.D, (size_t)c->nB * b * b));
@@ -334,20 +409,44 @@ int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, i
     ACHK(c->alloc(&c->d_rlog_off, (size_t)c->nB));
     ACHK(hipMemcpy(c->d_rlog_off, off.data(), c->nB * sizeof(int64_t), hipMemcpyHostToDevice));
   }
-  for (int s = 0; s < max_batch; ++s) ACHK(c->alloc(&c->slots[s].p.rlog, (size_t)c->nlog));
+  for (int s = 0; s < max_batch; ++s) {
+    ACHK(c->alloc(&c->slots[s].p.rlog, (size_t)c->nlog));
+    ACHK(c->alloc(&c->slots[s].p.rpl, (size_t)c->nlog));
+  }
   ACHK(c->alloc(&c->rep_rank0, (size_t)max_batch * nt));
   ACHK(c->alloc(&c->rep_rlog, (size_t)max_batch * c->nlog));
+  ACHK(c->alloc(&c->rep_rpl, (size_t)max_batch * c->nlog));
   ACHK(c->alloc(&c->d_stash_off, (size_t)max_batch * nt));
   ACHK(c->alloc(&c->d_replay, (size_t)max_batch));
-  ACHK(hipHostMalloc((void**)&c->h_replay, sizeof(int) * max_batch, hipHostMallocDefault));
   ACHK(c->alloc(&c->d_slots, (size_t)max_batch));
   ACHK(c->alloc(&c->d_retry, (size_t)max_batch));
   ACHK(c->alloc(&c->d_thr, (size_t)2 * max_batch));
-  ACHK(hipHostMalloc((void**)&c->h_thr, sizeof(double) * 2 * max_batch, hipHostMallocDefault));
+  if (c->arena) {   // pageable staging (the copies on the null stream complete before they return)
+    c->h_replay_host.assign(max_batch, -1);
+    c->h_thr_host.assign(2 * (size_t)max_batch, 0.0);
+    c->h_replay = c->h_replay_host.data();
+    c->h_thr = c->h_thr_host.data();
+  } else {
+    ACHK(hipHostMalloc((void**)&c->h_replay, sizeof(int) * max_batch, hipHostMallocDefault));
+    ACHK(hipHostMalloc((void**)&c->h_thr, sizeof(double) * 2 * max_batch, hipHostMallocDefault));
+  }
   std::vector<SlotPtrs> hp(max_batch);
   for (int s = 0; s < max_batch; ++s) hp[s] = c->slots[s].p;
   ACHK(hipMemcpy(c->d_slots, hp.data(), max_batch * sizeof(SlotPtrs), hipMemcpyHostToDevice));
 #undef ACHK
+  return TLR_OK;
+}
+
+extern "C" {
+
+int tlr_ctx_create(tlr_ctx** out, const double* S, const double* y, int64_t n, int32_t b, int32_t max_batch) {
+  if (!out) return set_err(TLR_ERR_ARG, "invalid argument");
+  tlr_ctx* c = new tlr_ctx();
+  const int rc = ctx_build(c, S, y, n, b, max_batch);
+  if (rc != TLR_OK) {
+    delete c;
+    return rc;
+  }
   *out = c;
   return TLR_OK;
 }
@@ -467,40 +566,60 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
 // streams are joined into the main stream.  Nothing the lookahead writes (D, rhs, the (., k+1) factors, W_i
 // and U_ik W_i of step k+1) is touched by the pairs (i, j), j > k+1, that may still be running, and the
 // lists of step k are complete before POTRF can record a failure.
+// Redo launches of step k on the main stream (panel != 0: the panel pairs' lists): FP32 first under BF16
+// (it may hand pairs on to FP64), then FP64.
+static hipError_t launch_redo(tlr_ctx* c, const SlotArgs& sa, int k, int panel) {
+  hipError_t e;
+  double* scr = panel ? c->scr_pan[3] : c->scr_cls[3];
+  if (sa.prec_mode == PM_BF16 &&
+      (e = launch_recomp_class(sa, k, 3, panel, c->recomp_grid[PM_FP32][3], scr, c->stream, PM_FP32, 2)) != hipSuccess)
+    return e;
+  return launch_recomp_class(sa, k, 3, panel, c->recomp_grid[PM_FP64][3], scr, c->stream, PM_FP64, 1);
+}
+
 static hipError_t launch_step_lookahead(tlr_ctx* c, const SlotArgs& sa, int k, int nslot) {
   hipError_t e;
   TimedLaunch tt{nullptr, nullptr, -1, k, 0.0, 0.0};
   const bool on = c->tic(5, tt);
+  // one stream (arena mode): every launch below runs in the order written, which respects each dependency
+  const bool ev = !c->single_stream;
   if ((e = launch_recomp_list(sa, k, nslot, c->stream)) != hipSuccess) return e;
-  if ((e = hipEventRecord(c->ev_list, c->stream)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecord(c->ev_list, c->stream)) != hipSuccess) return e;
   // classes run by side streams 1 and 2, in order; the third entry of stream 2 is a second launch over the
   // 17-32 lists (claims are atomic, so two launches share a list), which takes what stream 1 has not
   // started by the time stream 2 is done with its own classes
   static const int plan[2][3] = {{2, -1, -1}, {3, 1, 2}};
+  const int pm = sa.prec_mode;
+  const int* grid = c->recomp_grid[pm];
   for (int q = 1; q <= 2; ++q) {
     hipStream_t sq = c->cls_stream[q];
-    if ((e = hipStreamWaitEvent(sq, c->ev_list, 0)) != hipSuccess) return e;
+    if (ev && (e = hipStreamWaitEvent(sq, c->ev_list, 0)) != hipSuccess) return e;
     for (int pass = 1; pass >= 0; --pass) {   // panel pairs first, then the rest
       for (int u = 0; u < 3; ++u) {
         const int cls = plan[q - 1][u];
         if (cls < 0) continue;
         double* scr = u == 2 ? c->scr_help[pass] : (pass ? c->scr_pan[cls] : c->scr_cls[cls]);
-        if ((e = launch_recomp_class(sa, k, cls, pass, c->recomp_grid[cls], scr, sq)) != hipSuccess) return e;
+        if ((e = launch_recomp_class(sa, k, cls, pass, grid[cls], scr, sq, pm)) != hipSuccess) return e;
       }
-      if (pass == 1 && (e = hipEventRecord(c->ev_pan[q], sq)) != hipSuccess) return e;
+      if (ev && pass == 1 && (e = hipEventRecord(c->ev_pan[q], sq)) != hipSuccess) return e;
     }
-    if ((e = hipEventRecord(c->ev_cls[q], sq)) != hipSuccess) return e;
+    if (ev && (e = hipEventRecord(c->ev_cls[q], sq)) != hipSuccess) return e;
   }
   if ((e = launch_potrf(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
   if ((e = launch_trtri(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
-  if ((e = launch_recomp_class(sa, k, 0, 1, c->recomp_grid[0], c->scr_pan[0], c->stream)) != hipSuccess) return e;
-  if ((e = launch_recomp_class(sa, k, 0, 0, c->recomp_grid[0], c->scr_cls[0], c->stream)) != hipSuccess) return e;
-  for (int q = 1; q <= 2; ++q)
+  if ((e = launch_recomp_class(sa, k, 0, 1, grid[0], c->scr_pan[0], c->stream, pm)) != hipSuccess) return e;
+  if ((e = launch_recomp_class(sa, k, 0, 0, grid[0], c->scr_cls[0], c->stream, pm)) != hipSuccess) return e;
+  for (int q = 1; q <= 2 && ev; ++q)
     if ((e = hipStreamWaitEvent(c->stream, c->ev_pan[q], 0)) != hipSuccess) return e;
+  // reduced precision: the pairs the guard handed on, once every launch that could add to their lists is
+  // done (BF16 -> FP32 -> FP64); the widest class takes any pair, on the scratch of the finished panel (or
+  // rest) launches of that class
+  if (pm != PM_FP64 && (e = launch_redo(c, sa, k, 1)) != hipSuccess) return e;
   if ((e = launch_trsm(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
   if ((e = launch_diag_update(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
-  for (int q = 1; q <= 2; ++q)
+  for (int q = 1; q <= 2 && ev; ++q)
     if ((e = hipStreamWaitEvent(c->stream, c->ev_cls[q], 0)) != hipSuccess) return e;
+  if (pm != PM_FP64 && (e = launch_redo(c, sa, k, 0)) != hipSuccess) return e;
   c->toc(on, tt);
   return hipSuccess;
 }
@@ -592,7 +711,10 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
   if (rule != TLR_RULE_FIXED && rule != TLR_RULE_ADAPTIVE && rule != TLR_RULE_MIXED)
     return set_err(TLR_ERR_ARG, "unknown rule");
   if (rule == TLR_RULE_MIXED && replay) return set_err(TLR_ERR_ARG, "frozen-rank replay is not available for the mixed rule");
-  if (precision != TLR_PREC_FP64) return set_err(TLR_ERR_ARG, "unsupported precision");
+  if (precision != TLR_PREC_FP64 && precision != TLR_PREC_FP32 && precision != TLR_PREC_BF16)
+    return set_err(TLR_ERR_ARG, "unsupported precision");
+  if (rule == TLR_RULE_MIXED && precision != TLR_PREC_FP64)
+    return set_err(TLR_ERR_ARG, "the mixed storage rule runs with FP64 arithmetic only");
   for (int s = 0; s < m; ++s) {
     const double* th = thetas + 3 * s;
     if (!(th[0] > 0.0) || !(th[1] > 0.0) || !(th[2] >= 0.0)) return set_err(TLR_ERR_ARG, "theta must be positive");
@@ -601,11 +723,14 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
   hipStream_t st = c->stream;
   const int rule_fixed = rule == TLR_RULE_FIXED;
   const int mixed = rule == TLR_RULE_MIXED;
+  for (int s = 0; s < m; ++s) c->sstate[s].valid = false;
+  c->factorizations += m;
   c->ev_used = 0;
   c->timed.clear();
   HIPCHK(launch_fill_int(c->d_retry, c->max_batch, 0, st));
   HIPCHK(launch_fill_int(c->wl_count, (int64_t)WL_STRIDE * RCLASSES * c->nB, 0, st));
   HIPCHK(launch_fill_int(c->wlp_count, (int64_t)WL_STRIDE * RCLASSES * c->nB, 0, st));
+  HIPCHK(launch_fill_int(c->wlr_count, (int64_t)WL_STRIDE * 4 * c->nB, 0, st));
   // per-slot thresholds of the recompression (the adaptive budget depends on theta); the previous call
   // ended with a synchronisation, so the pinned staging buffer is free
   const double bmax_ = (double)(c->n < c->b ? c->n : c->b);
@@ -628,6 +753,7 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
     HIPCHK(launch_fill_int(p.status + ST_FAILED, 1, 0x7fffffff, st));
     HIPCHK(hipMemcpyAsync(p.rhs, c->y, c->n * sizeof(double), hipMemcpyDeviceToDevice, st));
     if (mixed) HIPCHK(launch_fill_int(p.prec, c->ntile, 64, st));
+    HIPCHK(launch_fill_int(p.rpl, c->nlog, -1, st));
   }
   int ncs = (c->timing_kernel >= 1 && c->timing_kernel <= 3) ? 1 : c->nset;
   if (ncs > m) ncs = m;
@@ -669,6 +795,7 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
   SlotArgs sa = c->args();
   sa.rule_fixed = rule_fixed;
   sa.mixed = mixed;
+  sa.prec_mode = precision == TLR_PREC_FP32 ? PM_FP32 : (precision == TLR_PREC_BF16 ? PM_BF16 : PM_FP64);
   HIPCHK(launch_potrf(sa, 0, m, st));
   HIPCHK(launch_trtri(sa, 0, m, st));
   HIPCHK(launch_trsm(sa, 0, m, st));
@@ -732,6 +859,17 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
     if (status) status[s] = ok ? TLR_OK : TLR_NOT_PD;
     if (failed_at) failed_at[s] = ok ? -1 : fa;
     if (!ok && m == 1) ret = TLR_NOT_PD;
+    if (!replay) {   // a plain evaluation: its factor stays resident in the slot until the next evaluation
+      SlotState& ss = c->sstate[s];
+      ss.valid = true;
+      memcpy(ss.theta, thetas + 3 * s, sizeof(ss.theta));
+      ss.rule = rule;
+      ss.value = value;
+      ss.precision = precision;
+      memcpy(ss.out3, out + 3 * s, sizeof(ss.out3));
+      ss.status = ok ? TLR_OK : TLR_NOT_PD;
+      ss.failed_at = ok ? -1 : fa;
+    }
     const int nB = c->nB;
     for (int which = 0; which < 2; ++which) {
       int32_t* dst = which == 0 ? ranks_sigma : ranks_factor;
@@ -794,6 +932,77 @@ int tlr_ctx_precisions(const tlr_ctx* ctx, int32_t slot, int32_t* prec) {
   return TLR_OK;
 }
 
+int tlr_set_error(int code, const char* msg) { return set_err(code, msg); }
+
+// tlr_loglik on a context whose every device buffer is carved from [base, base + bytes) (device memory
+// the caller owns), on the null stream, with persistent grids of at most grid_cap work groups; allocates
+// no device memory.  Used by the allocation-free entry point.
+int tlr_loglik_in_arena(void* base, size_t bytes, int grid_cap, const double* S, const double* y, int64_t n,
+                        int32_t b, const double theta[3], int32_t rule, double value, int32_t precision,
+                        double out3[3], int32_t* ranks_sigma, int32_t* ranks_factor, int32_t* failed_at) {
+  if (!base || bytes == 0 || grid_cap < 1) return set_err(TLR_ERR_ARG, "invalid workspace");
+  tlr_ctx c;
+  c.arena = (char*)base;
+  c.arena_cap = bytes;
+  c.grid_cap = grid_cap;
+  int rc = ctx_build(&c, S, y, n, b, 1);
+  if (rc != TLR_OK) {
+    if (c.arena_short)
+      return set_err(TLR_ERR_ARG, "N=" + std::to_string(n) + ", b=" + std::to_string(b) + " needs more than the " +
+                                      std::to_string(bytes >> 20) + " MiB of the static workspace");
+    return rc;
+  }
+  return tlr_loglik(&c, theta, rule, value, precision, out3, ranks_sigma, ranks_factor, failed_at);
+}
+
+int tlr_ctx_set_tunable(tlr_ctx* ctx, int32_t which, double value) {
+  if (!ctx) return set_err(TLR_ERR_ARG, "null context");
+  switch (which) {
+    case TLR_TUNE_GUARD:
+      if (!(value >= 1.0 && value <= 1e12)) return set_err(TLR_ERR_ARG, "guard must be in [1, 1e12]");
+      ctx->guard = value;
+      return TLR_OK;
+    case TLR_TUNE_DIAG_WG:
+      if (!(value >= 1.0 && value <= 64.0)) return set_err(TLR_ERR_ARG, "diag_wg must be in [1, 64]");
+      ctx->diag_wg = (int)value;
+      return TLR_OK;
+    case TLR_TUNE_GRID_FRAC:
+      // persistent recompression grids as a fraction of the resident capacity (scratch is sized for all of it)
+      if (!(value > 0.0 && value <= 1.0)) return set_err(TLR_ERR_ARG, "grid fraction must be in (0, 1]");
+      for (int pm = 0; pm < 3; ++pm)
+        for (int q = 0; q < RCLASSES; ++q) {
+          const int g = (int)(value * ctx->recomp_grid_max[pm][q]);
+          ctx->recomp_grid[pm][q] = g > 0 ? g : 1;
+        }
+      return TLR_OK;
+    default:
+      return set_err(TLR_ERR_ARG, "unknown tunable");
+  }
+}
+
+int tlr_ctx_get_tunable(const tlr_ctx* ctx, int32_t which, double* value) {
+  if (!ctx || !value) return set_err(TLR_ERR_ARG, "null argument");
+  switch (which) {
+    case TLR_TUNE_GUARD: *value = ctx->guard; return TLR_OK;
+    case TLR_TUNE_DIAG_WG: *value = ctx->diag_wg; return TLR_OK;
+    case TLR_TUNE_GRID_FRAC:
+      *value = (double)ctx->recomp_grid[0][0] / (ctx->recomp_grid_max[0][0] > 0 ? ctx->recomp_grid_max[0][0] : 1);
+      return TLR_OK;
+    default:
+      return set_err(TLR_ERR_ARG, "unknown tunable");
+  }
+}
+
+int tlr_ctx_arith_counts(const tlr_ctx* ctx, int32_t slot, int64_t counts[3]) {
+  if (!ctx || !counts || slot < 0 || slot >= ctx->max_batch) return set_err(TLR_ERR_ARG, "invalid argument");
+  std::vector<int> pv(ctx->nlog);
+  HIPCHK(hipMemcpy(pv.data(), ctx->slots[slot].p.rpl, ctx->nlog * sizeof(int), hipMemcpyDeviceToHost));
+  counts[0] = counts[1] = counts[2] = 0;
+  for (int v : pv)
+    if (v >= 0 && v < 3) ++counts[v];
+  return TLR_OK;
+}
+
 extern "C" {
 
 int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
@@ -818,6 +1027,8 @@ int tlr_loglik_grad_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
       HIPCHK(hipMemcpyAsync(c->rep_rank0 + (int64_t)s * c->ntile, p.rank0, c->ntile * sizeof(int),
                             hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->rep_rlog + (int64_t)s * c->nlog, p.rlog, c->nlog * sizeof(int), hipMemcpyDeviceToDevice,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(c->rep_rpl + (int64_t)s * c->nlog, p.rpl, c->nlog * sizeof(int), hipMemcpyDeviceToDevice,
                           c->stream));
   }
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -869,12 +1080,31 @@ int tlr_loglik_grad(tlr_ctx* ctx, const double theta[3], int32_t rule, double va
   return rc;
 }
 
+// The factor at theta in slot 0: the resident one when slot 0 holds exactly this evaluation (the factor handle
+// of the last tlr_loglik), else a new factorization.  out3 / status receive its result.
+static int factor_at(tlr_ctx* c, const double theta[3], int32_t rule, double value, int32_t precision, double out3[3],
+                     int32_t* status) {
+  if (c->sstate[0].matches(theta, rule, value, precision)) {
+    memcpy(out3, c->sstate[0].out3, 3 * sizeof(double));
+    *status = c->sstate[0].status;
+    return TLR_OK;
+  }
+  int32_t fa = -1;
+  return run_batch(c, 1, theta, rule, value, precision, nullptr, out3, status, &fa, nullptr, nullptr);
+}
+
+int tlr_ctx_stats(const tlr_ctx* ctx, int64_t* factorizations) {
+  if (!ctx) return set_err(TLR_ERR_ARG, "null context");
+  if (factorizations) *factorizations = ctx->factorizations;
+  return TLR_OK;
+}
+
 int tlr_predict(tlr_ctx* c, const double theta[3], int32_t rule, double value, int32_t precision, const double* Sp,
                 int64_t n_pred, double* mu, double* sd, double out3[3]) {
   if (!c || !theta || !Sp || n_pred < 1 || n_pred > 65535 || !mu || !sd) return set_err(TLR_ERR_ARG, "invalid argument");
   double o3[3];
-  int32_t st = 0, fa = -1;
-  int rc = run_batch(c, 1, theta, rule, value, precision, nullptr, o3, &st, &fa, nullptr, nullptr);
+  int32_t st = 0;
+  const int rc = factor_at(c, theta, rule, value, precision, o3, &st);
   if (rc < 0) return rc;
   if (out3) memcpy(out3, o3, sizeof(o3));
   if (st != TLR_OK) return TLR_NOT_PD;
@@ -927,8 +1157,8 @@ int tlr_solve(tlr_ctx* c, const double theta[3], int32_t rule, double value, int
               int64_t ldb, int64_t m, int32_t mode, double out3[3]) {
   if (!c || !theta || !B || m < 1 || ldb < c->n || mode < 0 || mode > 2) return set_err(TLR_ERR_ARG, "invalid argument");
   double o3[3];
-  int32_t st = 0, fa = -1;
-  int rc = run_batch(c, 1, theta, rule, value, precision, nullptr, o3, &st, &fa, nullptr, nullptr);
+  int32_t st = 0;
+  const int rc = factor_at(c, theta, rule, value, precision, o3, &st);
   if (rc < 0) return rc;
   if (out3) memcpy(out3, o3, sizeof(o3));
   if (st != TLR_OK) return TLR_NOT_PD;

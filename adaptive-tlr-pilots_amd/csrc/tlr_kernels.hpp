@@ -38,8 +38,20 @@ struct SlotPtrs {
   int* status;
   double* result;   // [loglik, logdet, quad]
   int* rlog;          // rank chosen by every recompression, at rlog_off[k] + pair index
+  int* rpl;           // arithmetic of every recompression (PM_FP64, PM_FP32, PM_BF16; -1 not run), as rlog
   int* prec;          // storage precision (8, 16, 32, 64) of every factor tile, mixed rule
 };
+
+// Arithmetic of the factorization's recompressions (the products W = V_ik^T V_jk and U_ik W, the projections,
+// Householder QRs, the Jacobi SVD of the core and the rebuilt factors): FP64; FP32; bfloat16 operands of the
+// two products with FP32 accumulation and FP32 for the rest.  POTRF, TRSM, the diagonal updates, the
+// initial compression and the reductions stay FP64 in every mode.
+enum { PM_FP64 = 0, PM_FP32 = 1, PM_BF16 = 2 };
+// A reduced-precision recompression goes ahead only when its budget is at least `guard` unit roundoffs of
+// its working precision times the scale of the core (the larger of ||core||_F and the bound ||U_ik W||_F
+// ||U_jk||_F of the update it absorbs); otherwise the pair goes to a redo list of a higher precision.
+constexpr double GUARD_DEFAULT = 16384.0;
+constexpr double EPS_FP32 = 1.1920928955078125e-07, EPS_BF16 = 3.90625e-03;   // 2^-23, 2^-8
 
 struct SlotArgs {
   int64_t n;
@@ -53,6 +65,13 @@ struct SlotArgs {
   int2* wlp;            // the same for the pairs (i, k+1) that update the next panel column (run first)
   int* wlp_count;
   int64_t wlp_cap;
+  int2* wlr;            // redo lists: (step, panel, level) -> entries; level 0 redo in FP64, 1 in FP32
+  int* wlr_count;       // nB x 2 (panel) x 2 (level) x WL_STRIDE counters
+  int64_t wlr_cap;
+  int prec_mode;        // PM_FP64, PM_FP32 or PM_BF16 (arithmetic of the class launches)
+  double guard;         // unit roundoffs of headroom a reduced-precision pair needs (GUARD_DEFAULT)
+  int diag_wg;          // work groups per tile of the diagonal update
+  const int* rep_rpl;   // records: arithmetic of each recompression, nlog per record (frozen in replays)
   int rule_fixed;       // 1: fixed relative tolerance, 0: adaptive absolute budget
   int mixed;            // 1: adaptive budget plus per-pair storage precision (rounded factors)
   const double* thr;    // per slot: (thr_rel, thr_abs)
@@ -94,9 +113,11 @@ hipError_t launch_trtri(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_trsm(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_recomp_list(const SlotArgs& sa, int k, int nslot, hipStream_t s);
-hipError_t launch_recomp_class(const SlotArgs& sa, int k, int cls, int panel, int grid, double* scr, hipStream_t s);
-// resident work groups per CU of each recompression class (b selects the row-panel variant)
-hipError_t recompress_occupancy(int b, int blocks_per_cu[RCLASSES]);
+// pm: arithmetic (PM_*); src: 0 the class's work list, 1 the FP64 redo list, 2 the FP32 redo list
+hipError_t launch_recomp_class(const SlotArgs& sa, int k, int cls, int panel, int grid, double* scr, hipStream_t s,
+                               int pm = PM_FP64, int src = 0);
+// resident work groups per CU of each recompression class in arithmetic pm (b selects the row-panel variant)
+hipError_t recompress_occupancy(int b, int pm, int blocks_per_cu[RCLASSES]);
 hipError_t launch_final(const SlotArgs& sa, int nslot, hipStream_t s);
 
 // solve.hip
@@ -107,3 +128,11 @@ hipError_t launch_col_dot(const double* X, int64_t ldx, int64_t n, int m, const 
                           hipStream_t s);
 
 }  // namespace tlr
+
+// tlr_api.cpp: the calling thread's tlr_last_error message; returns code.
+int tlr_set_error(int code, const char* msg);
+// tlr_api.cpp: tlr_loglik on a context carved from caller-owned device memory [base, base + bytes) (null
+// stream, persistent grids of at most grid_cap work groups, no device allocation).
+int tlr_loglik_in_arena(void* base, size_t bytes, int grid_cap, const double* S, const double* y, int64_t n,
+                        int32_t b, const double theta[3], int32_t rule, double value, int32_t precision,
+                        double out3[3], int32_t* ranks_sigma, int32_t* ranks_factor, int32_t* failed_at);

@@ -12,7 +12,9 @@ LIB_PATH = os.path.join(HERE, "libtlrgp.so")
 TLR_OK, TLR_NOT_PD = 0, 1
 TLR_ERR_ARG, TLR_ERR_HIP, TLR_ERR_RANKCAP, TLR_ERR_NODEVICE = -1, -2, -3, -4
 TLR_RULE_FIXED, TLR_RULE_ADAPTIVE, TLR_RULE_MIXED = 0, 1, 2
-TLR_PREC_FP64 = 0
+RULES_C = {"fixed": TLR_RULE_FIXED, "adaptive": TLR_RULE_ADAPTIVE, "mixed": TLR_RULE_MIXED}
+TLR_PREC_FP64, TLR_PREC_FP32, TLR_PREC_BF16 = 0, 1, 2
+TLR_TUNE_GUARD, TLR_TUNE_DIAG_WG, TLR_TUNE_GRID_FRAC = 0, 1, 2
 
 # every entry point of include/tlrgp.h with its ctypes signature
 _D = ctypes.POINTER(ctypes.c_double)
@@ -38,6 +40,10 @@ SIGNATURES = {
     "tlr_ctx_set_timing": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tlr_ctx_get_timing": (ctypes.c_int, [_VP, _D, _I64, _D, _D]),
     "tlr_ctx_precisions": (ctypes.c_int, [_VP, ctypes.c_int32, _I]),
+    "tlr_ctx_arith_counts": (ctypes.c_int, [_VP, ctypes.c_int32, _I64]),
+    "tlr_ctx_set_tunable": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_double]),
+    "tlr_ctx_stats": (ctypes.c_int, [_VP, _I64]),
+    "tlr_ctx_get_tunable": (ctypes.c_int, [_VP, ctypes.c_int32, _D]),
     "tlr_last_error": (ctypes.c_char_p, []),
     "tlr_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32]),
 }

@@ -8,6 +8,8 @@ Restated from the reference (semantics, not code):
                 covariance at burn/2, checkpoint every ckpt_every iterations with the RNG state
                                                          codes/38_tlr_cholesky_mcmc.py:89-126
   ess           initial-positive-sequence effective sample size      codes/19_tlr_mcmc_gpu.py:157-167
+  observed_information  the negative Hessian of log L in log theta and its standard errors, by second
+                differences of log L (codes/15_pilot_standard_errors.py:21-31) or by differencing gradients
 Added (no reference counterpart): fit_mle_grad (L-BFGS-B with the frozen-rank gradient) and hmc (leapfrog
 HMC over many chains, one batched log L + gradient evaluation per leapfrog step).
 
@@ -147,6 +149,65 @@ def fit_mle_grad(S, y, bounds, rule, value, x0, maxiter=100, h=1e-5):
     return dict(theta_hat=np.exp(res.x).tolist(), neg_loglik=float(res.fun), evals=trace["evals"],
                 failures=trace["failures"], seconds=time.time() - t0, success=bool(res.success),
                 grad=(-np.asarray(res.jac)).tolist())
+
+
+def observed_information(S, y, theta, bounds, rule, value, h=0.02, method="gradient", precision="fp64", ctx=None):
+    """Observed information (negative Hessian of log L in log theta) at theta, and its inverse's standard
+    errors: se_log = sqrt(diag(I^-1)), se_theta = theta * se_log (the delta method of 15:53-55).
+
+    method "gradient": central differences of the GPU gradient, H[:, i] = (g(x + h e_i) - g(x - h e_i)) / 2h,
+    symmetrised; the six gradients run as one batch (ranks re-chosen at each point, frozen within each
+    gradient).  method "loglik": the reference's scheme, second differences of log L with step h
+    (codes/15_pilot_standard_errors.py:21-31), its 19 evaluations in one batch.
+    Returns dict(info_log, se_log, se_theta, loglik) or None when an evaluation fails."""
+    from .likelihood import TLRContext, _tile_size_from_bounds
+    theta = np.asarray(theta, float)
+    x = np.log(theta)
+    p = 3
+    b = _tile_size_from_bounds(len(y), bounds)
+    if method == "gradient":
+        pts = [x + s * h * np.eye(p)[i] for i in range(p) for s in (1.0, -1.0)]
+        c = ctx if ctx is not None and ctx.max_batch >= len(pts) else TLRContext(S, y, b, max_batch=len(pts))
+        hg = 1e-5 if precision == "fp64" else 1e-3
+        r = c.loglik_grad_batch(np.exp(np.array(pts)), rule, value, h=hg, precision=precision, ranks=False)
+        if not r["completed"].all() or not np.isfinite(r["grad"]).all():
+            return None
+        H = np.empty((p, p))
+        for i in range(p):
+            H[:, i] = (r["grad"][2 * i] - r["grad"][2 * i + 1]) / (2 * h)
+        H = 0.5 * (H + H.T)
+        r0 = c.loglik_batch(theta[None, :], rule, value, precision, ranks=False)
+        ll = float(r0["loglik"][0])
+    elif method == "loglik":
+        pts = [x]
+        for i in range(p):
+            ei = h * np.eye(p)[i]
+            pts += [x + ei, x - ei]
+            for j in range(i):
+                ej = h * np.eye(p)[j]
+                pts += [x + ei + ej, x + ei - ej, x - ei + ej, x - ei - ej]
+        c = ctx if ctx is not None and ctx.max_batch >= len(pts) else TLRContext(S, y, b, max_batch=len(pts))
+        r = c.loglik_batch(np.exp(np.array(pts)), rule, value, precision, ranks=False)
+        if not r["completed"].all():
+            return None
+        f = r["loglik"]
+        H = np.empty((p, p))
+        q = 1
+        f0 = f[0]
+        for i in range(p):
+            fp, fm = f[q], f[q + 1]
+            q += 2
+            H[i, i] = (fp - 2 * f0 + fm) / h ** 2
+            for j in range(i):
+                H[i, j] = H[j, i] = (f[q] - f[q + 1] - f[q + 2] + f[q + 3]) / (4 * h ** 2)
+                q += 4
+        ll = float(f0)
+    else:
+        raise ValueError(f"unknown method {method!r}")
+    info = -H
+    cov = np.linalg.inv(info)
+    se_log = np.sqrt(np.diag(cov))
+    return dict(info_log=info, se_log=se_log, se_theta=theta * se_log, loglik=ll)
 
 
 def _save_ckpt(path, **kw):

@@ -23,7 +23,7 @@ from ._lib import check, dptr, f64, iptr, lib
 
 RULES = {"fixed": _lib.TLR_RULE_FIXED, "adaptive": _lib.TLR_RULE_ADAPTIVE, "mixed": _lib.TLR_RULE_MIXED}
 PRECISION_BITS = (8, 16, 32, 64)
-PRECISIONS = {"fp64": _lib.TLR_PREC_FP64}
+PRECISIONS = {"fp64": _lib.TLR_PREC_FP64, "fp32": _lib.TLR_PREC_FP32, "bf16": _lib.TLR_PREC_BF16}
 
 
 def tile_bounds(n, b):
@@ -172,6 +172,32 @@ class TLRContext:
         if rule == "mixed":
             res["precisions"] = np.stack([self.precisions(s) for s in range(m)])
         return res
+
+    TUNABLES = {"guard": _lib.TLR_TUNE_GUARD, "diag_wg": _lib.TLR_TUNE_DIAG_WG, "grid_frac": _lib.TLR_TUNE_GRID_FRAC}
+
+    def set_tunable(self, name, value):
+        """Set a launch / precision tunable: guard (headroom of the reduced-precision recompressions, in unit
+        roundoffs), diag_wg (work groups per tile of the diagonal update), grid_frac (persistent recompression
+        grids as a fraction of the resident capacity)."""
+        check(lib.tlr_ctx_set_tunable(self._ptr, self.TUNABLES[name], float(value)))
+
+    def get_tunable(self, name):
+        v = _lib.ctypes.c_double()
+        check(lib.tlr_ctx_get_tunable(self._ptr, self.TUNABLES[name], _lib.ctypes.byref(v)))
+        return v.value
+
+    def factorizations(self):
+        """Factorizations this context has run (a prediction or solve on the resident factor adds none)."""
+        v = _lib.ctypes.c_int64()
+        check(lib.tlr_ctx_stats(self._ptr, _lib.ctypes.byref(v)))
+        return v.value
+
+    def arith_counts(self, slot=0):
+        """Recompressions of the last evaluation in batch slot `slot` run in FP64, FP32 and BF16 (the
+        reduced-precision modes hand pairs whose budget is too tight on to a higher precision)."""
+        c = np.zeros(3, np.int64)
+        check(lib.tlr_ctx_arith_counts(self._ptr, int(slot), c.ctypes.data_as(_lib._I64)))
+        return dict(fp64=int(c[0]), fp32=int(c[1]), bf16=int(c[2]))
 
     def precisions(self, slot=0):
         """Storage precision in bits of every factor tile (nB x nB, 0 off the lower triangle) of the last

@@ -1,25 +1,31 @@
 #!/usr/bin/env python3
-"""Throughput of the adaptive-rank TLR log-likelihood on one MI355X.
+"""Throughput of the adaptive-rank TLR log-likelihood plus gradient on one MI355X, in reduced-precision
+arithmetic beside plain FP64.
 
-Workload (the plain FP64 TLR Cholesky with recompression at N = 40,000, b = 500): N = 40,000 sites
-i.i.d. uniform in the unit square (default_rng(0)), Morton-ordered, Matern nu=3/2, tiles b = 500
-(80 x 80 tile grid), adaptive rank rule tau = 0.9, ranks re-chosen at each of 20 theta on a Latin
-hypercube (scipy.stats.qmc.LatinHypercube(d=3, seed=0)) over sigma2 in [0.5, 2], range in [0.1, 0.6],
-eta in [0.01, 0.1].  One step = one batched evaluation of log L and its gradient in log theta at those 20
-theta (log L, log-det, quadratic form, per-tile ranks; gradient at frozen ranks: the sigma2 component in
-closed form, range and eta by central differences that replay the recorded ranks, i.e. four further
-evaluations per theta); value = log L + gradient evaluations per second.
+Workload: N = 40,000 sites i.i.d. uniform in the unit square (default_rng(0)), Morton-ordered, Matern
+nu=3/2, tiles b = 500 (80 x 80 tile grid), adaptive rank rule tau = 0.9, ranks re-chosen at each of 20 theta
+on a Latin hypercube (scipy.stats.qmc.LatinHypercube(d=3, seed=0)) over sigma2 in [0.5, 2], range in
+[0.1, 0.6], eta in [0.01, 0.1].  One step = one batched evaluation of log L and its gradient in log theta at
+those 20 theta (log L, log-det, quadratic form, per-tile ranks; gradient at frozen ranks and frozen
+arithmetic: the sigma2 component in closed form, range and eta by central differences that replay the
+recorded ranks, i.e. four further evaluations per theta); value = log L + gradient evaluations per second
+in the headline arithmetic (--precision, default plain FP64; FP32 and BF16 legs beside it: the
+factorization's off-diagonal products and recompressions in FP32 or with bfloat16 product operands, each
+pair guarded, POTRF / TRSM / diagonal updates / initial compression / log-det in FP64).
 
-Also reported: the roofline of the dominant kernel (the recompressions of each factorization step, timed
-with HIP events around each step's launches inside the timed region, algorithmic flops from the rank
-logs), the roofline of the compression's Matern products (one extra pass after the timed steps), and a
-CPU baseline (the NumPy/SciPy oracle timed on the host cores on a bounded sample of the same workload).
+Also reported: the same step in FP32 and in BF16 (bfloat16 product operands, FP32 accumulation) on the
+same theta, with their rates, the ratio to plain FP64, the largest |delta log L| and gradient differences
+against plain FP64 and the PD agreement; the roofline of the dominant kernel group (one factorization step: the recompressions of
+step k with the POTRF, inverse, TRSM and diagonal update of step k + 1, timed with HIP events inside the
+timed region, algorithmic flops and bytes from the rank logs); the compression's Matern products; and a
+CPU baseline (the NumPy/SciPy oracle timed on the host cores on one full evaluation at one theta).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -28,13 +34,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "adaptive-tlr-pilots_amd"))
 
-FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix), AMD published
-# HBM bytes per recompression step and theta (FETCH_SIZE + WRITE_SIZE of the step's dispatches for a
-# batch of 20 theta, averaged over the 78 steps and divided by 20, each counter calibrated against a device
-# copy of known size); measured with rocprofv3 --pmc on this workload, see
-# profiles/recompress_n40000_b500_pmc_traffic.md.  A launch group covers every theta of one batch (20 for
-# log L; with the gradient, 20 in the base batch and 80 in the replay batch, the same ranks per theta).
-RECOMPRESS_TRAFFIC_PER_STEP_THETA = 5.63e9 / 20
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 157.3}   # MI355X dense FP64 / FP32 (AMD published)
+HBM_PEAK_GBS = 8000.0
+# HBM bytes per factorization step and theta (FETCH_SIZE + WRITE_SIZE of the step's dispatches, each counter
+# calibrated against a device copy of known size, divided by the theta of the batch); measured with
+# rocprofv3 --pmc on this workload in plain FP64 (profiles/recompress_n40000_b500_pmc_traffic.md).
+STEP_TRAFFIC_PER_THETA = 5.63e9 / 20
+# central-difference step of the gradient in log theta: 1e-5 in FP64; the reduced-precision modes take a
+# larger step so that their rounding (not smooth in theta) stays small against the difference
+GRAD_H = {"fp64": 1e-5, "fp32": 1e-3, "bf16": 1e-3}
 
 
 def lhs_thetas(m, seed=0):
@@ -53,27 +61,41 @@ def load_field(n):
     return a[:, :2].copy(), a[:, 2].copy()
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(S, y, b, theta, rule, value, n_sample, evals_per_point):
-    """Time the CPU oracle (NumPy/SciPy restatement of the reference TLR Cholesky) on the first n_sample
-    sites of the same field (Morton order keeps them a compact sub-region), same b and rule; scaled to
-    the full problem by the ratio of algorithmic work (tile count for the compression and C(nB, 3) for the
-    recompressions), both measured separately."""
+    """Time the CPU oracle (NumPy/SciPy restatement of the reference TLR Cholesky, all host cores for BLAS)
+    on one full log L evaluation at theta (n_sample = N), or on the first n_sample sites of the field (Morton
+    order keeps them a compact sub-region) scaled to N by the algorithmic work (tile count for the
+    compression, C(nB, 3) for the recompressions); a log L + gradient costs evals_per_point evaluations (the
+    same frozen-rank difference scheme)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import tlr_oracle as O
     from threadpoolctl import threadpool_limits
     cores = os.cpu_count()
+    n_full = len(y)
     with threadpool_limits(limits=cores):
         Ss, ys = S[:n_sample], y[:n_sample]
         bnd = O.tile_bounds(n_sample, b)
         t0 = time.perf_counter()
         ll, _, info = O.tlr_loglik(Ss, ys, theta, bnd, rule, value)
         t = time.perf_counter() - t0
-    nbs = len(bnd)
-    nbf = (len(y) + b - 1) // b
-    # cost model: compression ~ tiles, factorization ~ recompressions C(nB-1, 2) summed over k ~ C(nB, 3)
-    tiles_s, tiles_f = nbs * (nbs - 1) / 2, nbf * (nbf - 1) / 2
-    rec_s, rec_f = nbs * (nbs - 1) * (nbs - 2) / 6, nbf * (nbf - 1) * (nbf - 2) / 6
-    # split the sample time with a second, compression-only timing
+    th = [float(x) for x in theta]
+    if n_sample >= n_full:
+        return dict(value=1.0 / (evals_per_point * t), unit="evals/s", cores=cores, kind="port", cpu=_cpu_model(),
+                    sample=(f"oracle tlr_loglik, one full evaluation at N={n_full}, b={b}, {rule} {value}, "
+                            f"theta={th}: {t:.1f} s (log L {ll}); x{evals_per_point} log L evaluations per "
+                            f"log L + gradient (the same frozen-rank difference scheme)"))
+    # smaller sample: split compression and factorization, scale each by its work
     with threadpool_limits(limits=cores):
         t0 = time.perf_counter()
         sig2, r, eta = theta
@@ -83,14 +105,41 @@ def cpu_baseline(S, y, b, theta, rule, value, n_sample, evals_per_point):
                 T = sig2 * O.matern32(O.dist(Ss[i0:i1], Ss[j0:j1]), r)
                 O.truncate_svd(T, rule, thr_rel, thr_abs)
         tc = time.perf_counter() - t0
-    tf = max(t - tc, 0.0)
-    t_full = tc * tiles_f / tiles_s + tf * rec_f / max(rec_s, 1.0)
-    return dict(value=1.0 / (evals_per_point * t_full), unit="evals/s", cores=cores, kind="port",
-                sample=(f"oracle tlr_loglik at N={n_sample}, b={b}, {rule} {value}, "
-                        f"theta={[float(x) for x in theta]}: {t:.1f} s ({tc:.1f} s compression); scaled to "
-                        f"N={len(y)} by tile count (compression) and C(nB,3) (recompressions): {t_full:.0f} s "
-                        f"per log L, x{evals_per_point} log L evaluations per log L + gradient (same "
-                        f"frozen-rank difference scheme)"))
+    nbs, nbf = len(bnd), (n_full + b - 1) // b
+    tiles_s, tiles_f = nbs * (nbs - 1) / 2, nbf * (nbf - 1) / 2
+    rec_s, rec_f = nbs * (nbs - 1) * (nbs - 2) / 6, nbf * (nbf - 1) * (nbf - 2) / 6
+    t_full = tc * tiles_f / tiles_s + max(t - tc, 0.0) * rec_f / max(rec_s, 1.0)
+    return dict(value=1.0 / (evals_per_point * t_full), unit="evals/s", cores=cores, kind="port", cpu=_cpu_model(),
+                sample=(f"oracle tlr_loglik at N={n_sample}, b={b}, {rule} {value}, theta={th}: {t:.1f} s "
+                        f"({tc:.1f} s compression), scaled to N={n_full} by tile count and C(nB,3): "
+                        f"{t_full:.0f} s per log L, x{evals_per_point} per log L + gradient"))
+
+
+def run_leg(ctx, thetas, prec, rule, value, steps, warmup, grad, timing=False, tag=""):
+    """warmup + steps batched evaluations in arithmetic prec; returns (seconds per step, last result,
+    timing of the factorization steps when timing)."""
+    h = GRAD_H[prec]
+
+    def step():
+        if grad:
+            return ctx.loglik_grad_batch(thetas, rule, value, h=h, precision=prec)
+        return ctx.loglik_batch(thetas, rule, value, precision=prec)
+
+    for w in range(warmup):
+        r = step()
+        print(f"# {tag}{prec} warmup {w}: completed {int(r['completed'].sum())}/{len(thetas)}", file=sys.stderr,
+              flush=True)
+    if timing:
+        ctx.set_timing(5)   # the factorization steps: the dominant kernel group
+    t0 = time.perf_counter()
+    for s in range(steps):
+        r = step()
+        print(f"# {tag}{prec} step {s}: {time.perf_counter() - t0:.3f} s", file=sys.stderr, flush=True)
+    elapsed = (time.perf_counter() - t0) / steps
+    tm = ctx.get_timing() if timing else None
+    if timing:
+        ctx.set_timing(0)
+    return elapsed, r, tm
 
 
 def main():
@@ -103,7 +152,13 @@ def main():
     ap.add_argument("--batch", type=int, default=20)
     ap.add_argument("--rule", default="adaptive")
     ap.add_argument("--value", type=float, default=0.9)
-    ap.add_argument("--cpu-sample", type=int, default=10000, help="sites of the CPU-baseline sample (0: skip)")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"],
+                    help="arithmetic of the headline leg (plain FP64: at this budget the guard lets FP32 take "
+                         "only a few per cent of the recompressions, so the reduced modes are not faster)")
+    ap.add_argument("--compare", default="fp32,bf16", help="comparison legs on the same theta ('' for none)")
+    ap.add_argument("--compare-steps", type=int, default=2, help="timed steps of each comparison leg")
+    ap.add_argument("--cpu-sample", type=int, default=-1,
+                    help="sites of the CPU-baseline sample (-1: the full N, one evaluation; 0: skip)")
     ap.add_argument("--no-grad", action="store_true", help="time log L alone")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -118,62 +173,89 @@ def main():
     # the gradient's 4 replays per theta run as one batch when the context holds 5 x batch slots
     # (about 1.8 GB per slot at N = 40,000, b = 500: 180 GB of the 288 GB for 20 theta)
     ctx = tlrgp.TLRContext(S, y, a.b, max_batch=a.batch * (5 if grad else 1))
-
-    def step():
-        if grad:
-            return ctx.loglik_grad_batch(thetas, a.rule, a.value)
-        return ctx.loglik_batch(thetas, a.rule, a.value)
-
-    for w in range(a.warmup):
-        r = step()
-        print(f"# warmup {w}: completed {int(r['completed'].sum())}/{a.batch}", file=sys.stderr, flush=True)
-    ctx.set_timing(5)   # recompressions of every factorization step: the dominant kernel
-    t0 = time.perf_counter()
-    for s in range(a.steps):
-        r = step()
-        print(f"# step {s}: {time.perf_counter() - t0:.3f} s", file=sys.stderr, flush=True)
-    elapsed = time.perf_counter() - t0
-    kms, launches, flops, kbytes = ctx.get_timing()
+    prec = a.precision
+    t_step, r, (kms, launches, flops, kbytes) = run_leg(ctx, thetas, prec, a.rule, a.value, a.steps, a.warmup,
+                                                        grad, timing=True)
+    counts = [ctx.arith_counts(s) for s in range(a.batch)]
+    legs = {}
+    for cp in [c for c in a.compare.split(",") if c and c != prec]:
+        tc, rc, _ = run_leg(ctx, thetas, cp, a.rule, a.value, a.compare_steps, 1, grad, tag="compare ")
+        legs[cp] = (tc, rc, [ctx.arith_counts(s) for s in range(a.batch)])
     # one more (untimed for throughput) pass timing the Matern products of the compression
     ctx.set_timing(1)
-    step()
+    ctx.loglik_batch(thetas, a.rule, a.value, precision=prec)
     pms, plaunches, pflops, pbytes = ctx.get_timing()
     ctx.set_timing(0)
-    ms_step = 1e3 * elapsed / a.steps
-    evals = a.batch * a.steps / elapsed
+    evals = a.batch / t_step
     nb = (a.n + a.b - 1) // a.b
     il = np.tril_indices(nb, -1)
     mean_rank = float(r["ranks_factor"][:, il[0], il[1]].mean())
-    achieved = flops / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
+    achieved_tf = flops / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
+    achieved_gbs = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    peak = PEAK_TFLOPS[prec]
+    frac_f, frac_b = achieved_tf / peak, achieved_gbs / HBM_PEAK_GBS
     # theta-steps per timed launch group (steps with recompressions: k = 0 .. nB - 3)
     theta_steps = a.batch * (5 if grad else 1) * a.steps * max(nb - 2, 0)
-    traffic = RECOMPRESS_TRAFFIC_PER_STEP_THETA * theta_steps / max(launches, 1)
+    traffic = STEP_TRAFFIC_PER_THETA * theta_steps / max(launches, 1)
+    if frac_b >= frac_f:
+        roof = {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": frac_b}
+    else:
+        roof = {"bound": "mfma", "achieved": achieved_tf, "peak": peak, "unit": "TFLOP/s", "frac": frac_f}
+    roof.update({"traffic": traffic,
+                 "kernel": "factorization step (work lists, k_recompress classes, lookahead POTRF/TRTRI/TRSM/"
+                           "diagonal update)",
+                 "launches": launches, "kernel_ms": kms, "flops_per_launch": flops / max(launches, 1),
+                 "bytes_per_launch": kbytes / max(launches, 1), "achieved_tflops": achieved_tf,
+                 "frac_of_peak_flops": frac_f, "achieved_gbs": achieved_gbs, "frac_of_hbm": frac_b,
+                 "limiter": "latency of the dependent per-pair steps (neither roofline is reached)"})
     p_achieved = pflops / (pms * 1e-3) / 1e12 if pms > 0 else 0.0
+    comparison = {}
+    for cp, (tc, rc, cc) in legs.items():
+        ok = r["completed"] & rc["completed"]
+        entry = {"evals_per_s": a.batch / tc, "ms_per_step": 1e3 * tc,
+                 "pd_agree": bool(np.array_equal(r["completed"], rc["completed"])),
+                 "completed": int(rc["completed"].sum()),
+                 "recompressions": {k: int(sum(c[k] for c in cc)) for k in ("fp64", "fp32", "bf16")}}
+        if ok.any():
+            entry["max_abs_dlogL_vs_headline"] = float(np.abs(r["loglik"][ok] - rc["loglik"][ok]).max())
+            if grad:
+                g1, g2 = r["grad"][ok], rc["grad"][ok]
+                entry["max_rel_dgrad_vs_headline"] = float(
+                    (np.abs(g1 - g2) / np.maximum(np.abs(g2), 1.0)).max())
+        comparison[cp] = entry
+    # mixed (reduced-precision) rate over plain FP64, whichever of them is the headline leg
+    if prec == "fp64":
+        mixed = comparison.get("fp32", {}).get("evals_per_s")
+        ratio = (mixed / evals) if mixed else None
+    else:
+        fp64_rate = comparison.get("fp64", {}).get("evals_per_s")
+        ratio = (evals / fp64_rate) if fp64_rate else None
     out = {
         "metric": "TLR logL+grad evals/s at N=40k, mixed vs plain FP64 GPU, |dlogL|<=1e-2",
         "value": evals, "unit": "evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (seed-0 uniform sites, Morton order, dense FP64 draw of y)",
+        "ms_per_step": 1e3 * t_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32+f64", "bf16": "bf16+f32+f64"}[prec],
+        "data": "synthetic (seed-0 uniform sites, Morton order, dense FP64 draw of y)",
         "config": {"workload": f"TLR log-likelihood{' + gradient' if grad else ''}, N={a.n}, b={a.b}, "
-                               f"{a.rule} tau={a.value}, {a.batch} Latin-hypercube theta per step, plain FP64",
-                   "n": a.n, "b": a.b, "theta_per_step": a.batch, "precision": "fp64", "gradient": grad,
-                   "gradient_method": "frozen ranks: sigma2 closed form, range/eta central differences h=1e-5"
-                   if grad else None,
-                   "mixed_precision": "not used: FP32 products measured 2.3x slower with a per-tile FP64 fallback "
-                                      "and |dlogL| up to 4.8e-2 at this configuration",
-                   "completed": int(r["completed"].sum()), "mean_rank_factor": mean_rank},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "k_recompress (work lists + 4 width classes, one launch group per step)",
-                     "launches": launches, "kernel_ms": kms,
-                     "flops_per_launch": flops / max(launches, 1)},
-        "roofline_compression": {"bound": "mfma", "achieved": p_achieved, "peak": FP64_PEAK_TFLOPS,
-                                 "unit": "TFLOP/s", "frac": p_achieved / FP64_PEAK_TFLOPS,
-                                 "kernel": "k_matern_product", "launches": plaunches, "kernel_ms": pms},
+                               f"{a.rule} tau={a.value}, {a.batch} Latin-hypercube theta per step, "
+                               f"off-diagonal arithmetic {prec}",
+                   "n": a.n, "b": a.b, "theta_per_step": a.batch, "precision": prec, "gradient": grad,
+                   "gradient_method": f"frozen ranks and arithmetic: sigma2 closed form, range/eta central "
+                                      f"differences h={GRAD_H[prec]}" if grad else None,
+                   "completed": int(r["completed"].sum()), "mean_rank_factor": mean_rank,
+                   "recompressions": {k: int(sum(c[k] for c in counts)) for k in ("fp64", "fp32", "bf16")}},
+        "mixed_vs_plain_fp64": ratio,
+        "comparison": comparison,
+        "roofline": roof,
+        "roofline_compression": {"bound": "mfma", "achieved": p_achieved, "peak": PEAK_TFLOPS["fp64"],
+                                 "unit": "TFLOP/s", "frac": p_achieved / PEAK_TFLOPS["fp64"],
+                                 "kernel": "k_matern_product", "launches": plaunches, "kernel_ms": pms,
+                                 "bytes_per_launch": pbytes / max(plaunches, 1),
+                                 "achieved_gbs": pbytes / (pms * 1e-3) / 1e9 if pms > 0 else 0.0},
     }
     if a.cpu_sample:
-        out["cpu_baseline"] = cpu_baseline(S, y, a.b, tuple(thetas[0]), a.rule, a.value, a.cpu_sample,
-                                           5 if grad else 1)
+        n_s = a.n if a.cpu_sample < 0 else a.cpu_sample
+        out["cpu_baseline"] = cpu_baseline(S, y, a.b, tuple(thetas[0]), a.rule, a.value, n_s, 5 if grad else 1)
     print(json.dumps(out), flush=True)
 
 

@@ -47,7 +47,20 @@ extern "C" {
 #define TLR_RULE_ADAPTIVE 1
 #define TLR_RULE_MIXED 2
 
+/* Arithmetic of the factorization's off-diagonal low-rank work: the products W = V_ik^T V_jk and U_ik W of
+ * every trailing update and the recompression of the updated tile (QR of both concatenations, SVD of the
+ * core, rebuilt factors; 17:119-133, 187-205).  POTRF, TRSM, the diagonal updates, the initial compression
+ * with its rank decision and the log-determinant stay FP64 in every mode.
+ *  TLR_PREC_FP64  plain FP64;
+ *  TLR_PREC_FP32  FP32 throughout the recompression;
+ *  TLR_PREC_BF16  the two products with bfloat16 operands and FP32 accumulation, the rest in FP32.
+ * A pair whose budget is below `guard` (16384 by default, TLR_TUNE_GUARD) unit roundoffs of the working precision times the core's scale (the larger
+ * of ||core||_F and ||U_ik W||_F ||U_jk||_F) is recompressed at the next higher precision instead (BF16 ->
+ * FP32 -> FP64).  The frozen-rank replays of the gradient keep each pair's arithmetic.  The storage rule
+ * TLR_RULE_MIXED runs with TLR_PREC_FP64 only.  No counterpart in the reference (its arithmetic is FP64). */
 #define TLR_PREC_FP64 0
+#define TLR_PREC_FP32 1
+#define TLR_PREC_BF16 2
 
 typedef struct tlr_ctx tlr_ctx;
 
@@ -92,19 +105,25 @@ int tlr_loglik_grad(tlr_ctx* ctx, const double theta[3], int32_t rule, double va
                     double out3[3], double grad3[3], int32_t* ranks_sigma, int32_t* ranks_factor, int32_t* failed_at);
 
 /* Kriging prediction through the TLR factor at theta: replaces tlr_predict (17:269-276) with its solves
- * tlr_forward / tlr_backward (17:230-256).  The factorization is evaluated at theta first (out3 receives its
- * loglik, logdet, quad; may be NULL).  Sp is n_pred x 2 row-major (host); mu and sd (n_pred, host) receive
+ * tlr_forward / tlr_backward (17:230-256).  The factor is the one the last evaluation of batch slot 0 left
+ * resident when that was this theta, rule, value and precision (the factor handle of tlr_loglik, as the
+ * reference passes (L_diag, L_off) from tlr_loglik to tlr_predict), else a new factorization at theta (out3
+ * receives its loglik, logdet, quad; may be NULL).  Sp is n_pred x 2 row-major (host); mu and sd (n_pred, host) receive
  * Kp L^{-T} L^{-1} y and sqrt(max(sigma2 (1 + eta) - colsum((L^{-1} Kp^T)^2), 1e-12)).  Returns TLR_NOT_PD
  * when the factorization fails. */
 int tlr_predict(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, int32_t precision, const double* Sp,
                 int64_t n_pred, double* mu, double* sd, double out3[3]);
 
-/* Factorize at theta and solve with the TLR factor, in place on the host block B (N x m, column-major,
+/* Solve with the TLR factor at theta (resident or new, as for tlr_predict), in place on the host block B (N x m, column-major,
  * leading dimension ldb >= N): mode 0 B <- L^{-1} B (tlr_forward, 17:230-240), mode 1 B <- L^{-T} B
  * (tlr_backward, 17:243-256), mode 2 B <- (L L^T)^{-1} B.  out3 receives (loglik, logdet, quad) of the
  * factorization (may be NULL).  Returns TLR_NOT_PD when the factorization fails (B unchanged). */
 int tlr_solve(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, int32_t precision, double* B,
               int64_t ldb, int64_t m, int32_t mode, double out3[3]);
+
+/* Factorizations a context has run (every theta of every batch, replays included): shows when tlr_predict /
+ * tlr_solve reused a resident factor.  No counterpart in the reference. */
+int tlr_ctx_stats(const tlr_ctx* ctx, int64_t* factorizations);
 
 /* Timing of one kernel class with HIP events on the context's stream, for roofline reports (no
  * counterpart in the reference, which times whole evaluations with time.time(), 17:359-362).
@@ -123,6 +142,22 @@ int tlr_ctx_get_timing(const tlr_ctx* ctx, double* total_ms, int64_t* launches, 
  * `slot` under TLR_RULE_MIXED: prec is nB x nB int32 row-major, entry [i * nB + j] for the tile (i, j),
  * 0 elsewhere.  Replaces PREC_OF and info["precision_mix_factor_tiles"] of 17:38, 210-214. */
 int tlr_ctx_precisions(const tlr_ctx* ctx, int32_t slot, int32_t* prec);
+
+/* Tunables of a context (no counterpart in the reference; set by tlrgp.autotune):
+ *  TLR_TUNE_GUARD      unit roundoffs of headroom a reduced-precision recompression needs (default 16384):
+ *                      budget >= guard * u * scale, u = 2^-23 (FP32) or 2^-8 (BF16);
+ *  TLR_TUNE_DIAG_WG    work groups per tile of the diagonal update (default 9);
+ *  TLR_TUNE_GRID_FRAC  persistent recompression grids as a fraction (0, 1] of the resident capacity. */
+#define TLR_TUNE_GUARD 0
+#define TLR_TUNE_DIAG_WG 1
+#define TLR_TUNE_GRID_FRAC 2
+int tlr_ctx_set_tunable(tlr_ctx* ctx, int32_t which, double value);
+int tlr_ctx_get_tunable(const tlr_ctx* ctx, int32_t which, double* value);
+
+/* Recompressions of the last evaluation in batch slot `slot` by arithmetic: counts[0] FP64, counts[1] FP32,
+ * counts[2] BF16 (pairs with r_ik = 0 or r_jk = 0 are not recompressed and not counted).  No counterpart in
+ * the reference. */
+int tlr_ctx_arith_counts(const tlr_ctx* ctx, int32_t slot, int64_t counts[3]);
 
 /* Message of the last error on the calling thread. */
 const char* tlr_last_error(void);

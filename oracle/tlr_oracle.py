@@ -232,21 +232,29 @@ def _svd(A, full):
         return _scipy_svd(A, full_matrices=full, lapack_driver="gesvd")
 
 
-def truncate_svd(T, rule, thr_rel, thr_abs):
-    """Rank-r truncation of a dense tile; T ~ U V^T with U = U_r s_r, V = V_r.  Follows 17:107-116."""
+def truncate_svd(T, rule, thr_rel, thr_abs, rank=None):
+    """Rank-r truncation of a dense tile; T ~ U V^T with U = U_r s_r, V = V_r.  Follows 17:107-116.
+    rank: use this rank instead of the rule's (frozen-rank replay; not in the reference)."""
     U, s, Vt = _svd(T, False)
     r, err = _rank_of(s, rule, thr_rel, thr_abs)
+    if rank is not None:
+        r = min(int(rank), len(s))
+        err = float(np.sqrt((s[r:] ** 2).sum()))
     return U[:, :r] * s[:r], Vt[:r].T, err
 
 
-def recompress(U, V, rule, thr_rel, thr_abs):
-    """Smallest-rank re-truncation of U V^T via QR(U), QR(V) and the SVD of R_u R_v^T.  Follows 17:119-133."""
+def recompress(U, V, rule, thr_rel, thr_abs, rank=None):
+    """Smallest-rank re-truncation of U V^T via QR(U), QR(V) and the SVD of R_u R_v^T.  Follows 17:119-133.
+    rank: as in truncate_svd."""
     if U.shape[1] == 0:
         return U, V, 0.0
     Qu, Ru = np.linalg.qr(U)
     Qv, Rv = np.linalg.qr(V)
     Uw, s, Vwt = _svd(Ru @ Rv.T, True)
     r, err = _rank_of(s, rule, thr_rel, thr_abs)
+    if rank is not None:
+        r = min(int(rank), len(s))
+        err = float(np.sqrt((s[r:] ** 2).sum()))
     return Qu @ (Uw[:, :r] * s[:r]), Qv @ Vwt[:r].T, err
 
 
@@ -314,12 +322,14 @@ def round_pair(U, V, budget):
 
 
 # ----------------------------------------------------------------------------- TLR Cholesky
-def tlr_cholesky(S, theta, bounds, rule, value):
+def tlr_cholesky(S, theta, bounds, rule, value, frozen=None):
     """Right-looking TLR Cholesky of Sigma(theta) without forming it.  Follows 17:137-215.
 
     Returns (L_diag, L_off, info); (None, None, info) with completed=False, failed_at=k when the
     diagonal tile k is not positive definite.  info adds 'ranks_sigma' and 'ranks_factor' (nB x nB
-    int arrays, lower triangle) to the reference's keys."""
+    int arrays, lower triangle) to the reference's keys, and 'rank_log' ({(k, i, j): rank} of every
+    recompression).  frozen: the info of an earlier evaluation whose ranks_sigma and rank_log are replayed
+    instead of the rule's choices (frozen ranks, for derivatives; not in the reference)."""
     n = S.shape[0]
     nB = len(bounds)
     b = max(e - s for s, e in bounds)
@@ -340,14 +350,16 @@ def tlr_cholesky(S, theta, bounds, rule, value):
         stored0 += (i1 - i0) ** 2
         for j, (j0, j1) in enumerate(bounds[:i]):
             T = sigma2 * matern32(dist(S[i0:i1], S[j0:j1]), r)
-            U, V, tail = truncate_svd(T, rule, thr_rel, thr_abs)
+            U, V, tail = truncate_svd(T, rule, thr_rel, thr_abs,
+                                      None if frozen is None else frozen["ranks_sigma"][i, j])
             if mixed:
                 U, V, _ = round_pair(U, V, thr_abs - tail)
             A_off[(i, j)] = (U, V)
             ranks0[i, j] = U.shape[1]
             stored0 += 2 * U.shape[1] * max(i1 - i0, j1 - j0)
+    rank_log = {}
     info = dict(stored_sigma=int(stored0), max_rank_sigma=int(ranks0.max()) if nB > 1 else 0,
-                ranks_sigma=ranks0)
+                ranks_sigma=ranks0, rank_log=rank_log)
     L_diag, L_off = [None] * nB, {}
     for k in range(nB):
         try:
@@ -379,7 +391,9 @@ def tlr_cholesky(S, theta, bounds, rule, value):
                 else:
                     Uij, Vij = A_off[(i, j)]
                     Ur, Vr, tail = recompress(np.hstack([Uij, -(Ui @ W)]), np.hstack([Vij, Uj]),
-                                              rule, thr_rel, thr_abs)
+                                              rule, thr_rel, thr_abs,
+                                              None if frozen is None else frozen["rank_log"][(k, i, j)])
+                    rank_log[(k, i, j)] = Ur.shape[1]
                     if mixed:
                         Ur, Vr, _ = round_pair(Ur, Vr, thr_abs - tail)
                     A_off[(i, j)] = (Ur, Vr)
@@ -399,6 +413,25 @@ def tlr_cholesky(S, theta, bounds, rule, value):
                     precision_mix_factor_tiles={p: int(sum(1 for k in L_off if prec[k] == p)) for p in PRECISIONS},
                     precision_factor=prec)
     return L_diag, L_off, info
+
+
+def tlr_loglik_grad_fd(S, y, theta, bounds, rule, value, h=1e-5):
+    """log L and its gradient in (log sigma2, log range, log eta) at frozen ranks: the ranks chosen at theta
+    are replayed at theta * exp(+-h e_k) and the central difference is taken (the derivative of a smooth
+    function of theta).  Returns (ll, grad, info) or (None, None, info).  Not in the reference (its
+    standard errors difference log L with re-chosen ranks, codes/15_pilot_standard_errors.py:21-31)."""
+    theta = np.asarray(theta, float)
+    ll, _, info = tlr_loglik(S, y, theta, bounds, rule, value)
+    if ll is None:
+        return None, None, info
+    g = np.empty(3)
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = h
+        lp = tlr_loglik(S, y, theta * np.exp(e), bounds, rule, value, frozen=info)[0]
+        lm = tlr_loglik(S, y, theta * np.exp(-e), bounds, rule, value, frozen=info)[0]
+        g[k] = np.nan if lp is None or lm is None else (lp - lm) / (2 * h)
+    return ll, g, info
 
 
 def tlr_forward(L_diag, L_off, bounds, B):
@@ -442,10 +475,10 @@ def tlr_matvec_L(L_diag, L_off, bounds, Z):
     return Y
 
 
-def tlr_loglik(S, y, theta, bounds, rule, value):
+def tlr_loglik(S, y, theta, bounds, rule, value, frozen=None):
     """TLR log-likelihood; (ll, (L_diag, L_off), info) or (None, None, info).  Follows 17:259-266.
-    info additionally carries 'logdet' and 'quad' (z^T z)."""
-    L_diag, L_off, info = tlr_cholesky(S, theta, bounds, rule, value)
+    info additionally carries 'logdet' and 'quad' (z^T z); frozen as in tlr_cholesky."""
+    L_diag, L_off, info = tlr_cholesky(S, theta, bounds, rule, value, frozen)
     if L_diag is None:
         return None, None, info
     z = tlr_forward(L_diag, L_off, bounds, y[:, None])[:, 0]

@@ -144,3 +144,26 @@ def test_n40000_b500_vs_oracle(tlrgp):
     assert _close(ll, float(ref["loglik"])), (ll, float(ref["loglik"]))
     assert _close(info["logdet"], float(ref["logdet"]))
     assert _close(info["quad"], float(ref["quad"]))
+
+
+def test_n40000_b500_lhs_thetas_vs_oracle(tlrgp):
+    """Three of the benchmark's own Latin-hypercube theta (N=40,000, b=500, adaptive 0.9, seed-0 field), in one
+    batch: every initial and final tile rank identical to the oracle's and log L / log-det / quadratic form
+    within 1e-6 relative (tests/golden/make_seed0_lhs_oracle.py)."""
+    path = os.path.join(GOLDEN, "seed0_n40000_b500_lhs_oracle.npz")
+    if not os.path.exists(path):
+        pytest.skip("oracle fixture not generated")
+    ref = np.load(path)
+    S, y = _seed0_field(40000)
+    th = ref["theta"]
+    print(f"case N=40000 b=500 adaptive:0.9 seed0, LHS rows {ref['rows'].tolist()}", flush=True)
+    ctx = tlrgp.TLRContext(S, y, 500, max_batch=len(th))
+    r = ctx.loglik_batch(th, "adaptive", 0.9)
+    ctx.close()
+    assert r["completed"].all()
+    for m in range(len(th)):
+        np.testing.assert_array_equal(r["ranks_sigma"][m], ref["ranks_sigma"][m])
+        np.testing.assert_array_equal(r["ranks_factor"][m], ref["ranks_factor"][m])
+        assert _close(r["loglik"][m], float(ref["loglik"][m])), (m, r["loglik"][m], float(ref["loglik"][m]))
+        assert _close(r["logdet"][m], float(ref["logdet"][m]))
+        assert _close(r["quad"][m], float(ref["quad"][m]))

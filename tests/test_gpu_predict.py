@@ -60,3 +60,29 @@ def test_predict_skill_goldens(tlrgp, goldens, n):
         sk = tlrgp.skill(hold[:, 2], mu, sd)
         for key in ("rmse", "crps", "cover90"):
             assert abs(sk[key] - r["skill_true_theta"][key]) < 1e-8, (r["rule"], r["value"], key, sk[key])
+
+
+def test_predict_reuses_resident_factor(tlrgp):
+    """tlr_predict with the factor tlr_loglik returned runs no second factorization (the factor handle), and
+    gives the same values as a prediction that has to factorize again (another theta evaluated in between)."""
+    S, y = _field(2000)
+    Sp = np.random.default_rng(6).random((40, 2))
+    th = np.array([1.1, 0.25, 0.04])
+    bnd = tlrgp.tile_bounds(2000, 250)
+    print("case predict factor handle N=2000 b=250", flush=True)
+    ll, fac, info = tlrgp.tlr_loglik(S, y, th, bnd, "adaptive", 0.9)
+    ctx = fac.ctx
+    n0 = ctx.factorizations()
+    mu, sd = tlrgp.tlr_predict(S, y, Sp, th, bnd, fac)
+    assert ctx.factorizations() == n0
+    ctx.loglik(np.array([0.9, 0.3, 0.05]), "adaptive", 0.9)   # slot 0 now holds another theta
+    n1 = ctx.factorizations()
+    mu2, sd2 = tlrgp.tlr_predict(S, y, Sp, th, bnd, fac)
+    assert ctx.factorizations() == n1 + 1
+    np.testing.assert_array_equal(mu, mu2)
+    np.testing.assert_array_equal(sd, sd2)
+    # solves reuse it too
+    n2 = ctx.factorizations()
+    z, o3 = ctx.solve(th, "adaptive", 0.9, y, "forward")
+    assert ctx.factorizations() == n2
+    assert abs(float(z @ z) - o3[2]) <= 1e-9 * o3[2]

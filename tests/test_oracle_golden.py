@@ -308,3 +308,25 @@ def test_n10000_mixed_rule_vs_reference(goldens):
         assert abs(info["precision_mix_factor_tiles"][int(p)] - cnt) <= 10, p
     assert abs(info["stored_factor"] / g["stored_factor"] - 1) < 5e-3
     assert abs(info["stored_factor_bytes"] / g["stored_factor_bytes"] - 1) < 5e-3
+
+
+# ------------------------------------------------------------------ frozen-rank replay
+def test_frozen_replay_reproduces_and_differentiates():
+    """Replaying an evaluation's own ranks at the same theta reproduces it exactly; the frozen-rank central
+    difference of log L in log sigma2 equals the closed form quad / 2 - N / 2 (tiles, budgets and hence the
+    ranks scale together with sigma2), and the other components approach the dense analytic gradient as the
+    rule tightens."""
+    S, y = O.simulate_seed0(400, seed=9)
+    th = np.array([1.1, 0.25, 0.04])
+    bnd = O.tile_bounds(400, 100)
+    ll, _, info = O.tlr_loglik(S, y, th, bnd, "adaptive", 0.9)
+    ll2, _, info2 = O.tlr_loglik(S, y, th, bnd, "adaptive", 0.9, frozen=info)
+    assert ll2 == ll
+    np.testing.assert_array_equal(info2["ranks_factor"], info["ranks_factor"])
+    assert info2["rank_log"] == info["rank_log"]
+    ll3, g, info3 = O.tlr_loglik_grad_fd(S, y, th, bnd, "adaptive", 0.9)
+    assert ll3 == ll
+    assert abs(g[0] - (0.5 * info["quad"] - 0.5 * 400)) < 1e-5 * abs(g[0])
+    _, gt, _ = O.tlr_loglik_grad_fd(S, y, th, bnd, "fixed", 1e-9)
+    _, gd = O.loglik_dense_grad(S, y, th)
+    np.testing.assert_allclose(gt, gd, rtol=1e-5, atol=1e-5)
