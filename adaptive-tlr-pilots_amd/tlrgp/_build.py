@@ -7,7 +7,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 LIB = os.path.join(HERE, "libtlrgp.so")
-SOURCES = ["compress.hip", "factor.hip", "solve.hip", "tlr_api.cpp"]
+SOURCES = ["compress.hip", "factor.hip", "solve.hip", "noalloc.hip", "tlr_api.cpp"]
 ARCH = "gfx950"
 # -packed-fp32-ops: no v_pk_*_f32 in the device code (the host pass ignores the feature with a warning)
 FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wno-unused-result",

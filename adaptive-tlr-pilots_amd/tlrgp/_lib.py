@@ -44,6 +44,8 @@ SIGNATURES = {
     "tlr_ctx_set_tunable": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_double]),
     "tlr_ctx_stats": (ctypes.c_int, [_VP, _I64]),
     "tlr_ctx_get_tunable": (ctypes.c_int, [_VP, ctypes.c_int32, _D]),
+    "tlr_loglik_noalloc": (ctypes.c_int, [_D, _D, ctypes.c_int64, ctypes.c_int32, _D, ctypes.c_int32, ctypes.c_double,
+                                          ctypes.c_int32, _D, _I, _I, _I]),
     "tlr_last_error": (ctypes.c_char_p, []),
     "tlr_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32]),
 }

@@ -159,6 +159,21 @@ int tlr_ctx_get_tunable(const tlr_ctx* ctx, int32_t which, double* value);
  * the reference. */
 int tlr_ctx_arith_counts(const tlr_ctx* ctx, int32_t slot, int64_t counts[3]);
 
+/* tlr_loglik without a context and without any device allocation: no counterpart in the reference; for
+ * callers that cannot call hipMalloc (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  Same arguments and results as tlr_loglik plus the field (S, y host pointers, N, b as
+ * in tlr_ctx_create).  Its device workspace is a static array of TLR_NOALLOC_BYTES per GPU, allocated when
+ * the library is loaded and held until the process ends; N <= TLR_NOALLOC_MAX_N and b <= TLR_NOALLOC_MAX_B
+ * (larger problems return TLR_ERR_ARG).  Runs on the null stream and returns after a synchronisation; calls
+ * on one GPU share the array, so this process serialises them, and calls that overlap on one GPU from
+ * elsewhere would race. */
+#define TLR_NOALLOC_BYTES (64ull << 20)
+#define TLR_NOALLOC_MAX_N 2048
+#define TLR_NOALLOC_MAX_B 512
+int tlr_loglik_noalloc(const double* S, const double* y, int64_t n, int32_t b, const double theta[3], int32_t rule,
+                       double value, int32_t precision, double out3[3], int32_t* ranks_sigma, int32_t* ranks_factor,
+                       int32_t* failed_at);
+
 /* Message of the last error on the calling thread. */
 const char* tlr_last_error(void);
 
