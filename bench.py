@@ -156,7 +156,9 @@ def main():
                     help="arithmetic of the headline leg (plain FP64: at this budget the guard lets FP32 take "
                          "only a few per cent of the recompressions, so the reduced modes are not faster)")
     ap.add_argument("--compare", default="fp32,bf16", help="comparison legs on the same theta ('' for none)")
-    ap.add_argument("--compare-steps", type=int, default=2, help="timed steps of each comparison leg")
+    ap.add_argument("--compare-steps", type=int, default=1, help="timed steps of each comparison leg")
+    ap.add_argument("--compare-warmup", type=int, default=0,
+                    help="untimed steps of each comparison leg (the headline leg has loaded every kernel)")
     ap.add_argument("--cpu-sample", type=int, default=-1,
                     help="sites of the CPU-baseline sample (-1: the full N, one evaluation; 0: skip)")
     ap.add_argument("--no-grad", action="store_true", help="time log L alone")
@@ -179,7 +181,8 @@ def main():
     counts = [ctx.arith_counts(s) for s in range(a.batch)]
     legs = {}
     for cp in [c for c in a.compare.split(",") if c and c != prec]:
-        tc, rc, _ = run_leg(ctx, thetas, cp, a.rule, a.value, a.compare_steps, 1, grad, tag="compare ")
+        tc, rc, _ = run_leg(ctx, thetas, cp, a.rule, a.value, a.compare_steps, a.compare_warmup, grad,
+                            tag="compare ")
         legs[cp] = (tc, rc, [ctx.arith_counts(s) for s in range(a.batch)])
     # one more (untimed for throughput) pass timing the Matern products of the compression
     ctx.set_timing(1)

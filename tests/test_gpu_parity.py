@@ -167,3 +167,30 @@ def test_n40000_b500_lhs_thetas_vs_oracle(tlrgp):
         assert _close(r["loglik"][m], float(ref["loglik"][m])), (m, r["loglik"][m], float(ref["loglik"][m]))
         assert _close(r["logdet"][m], float(ref["logdet"][m]))
         assert _close(r["quad"][m], float(ref["quad"][m]))
+
+
+@pytest.mark.parametrize("n", [60000, 100000])
+def test_largest_reference_runs(tlrgp, goldens, n):
+    """The reference's two largest runs (b=250, theta=(1, 0.3, 0.05); N=100,000 is 79,800 off-diagonal tiles):
+    the adaptive rule's log L, stored values and mean/max factor rank, and the fixed 1e-4 rule's failing tile,
+    against tlr_cholesky_n{60000,100000}.json."""
+    S, y = _golden_field(n)
+    th = np.array([1.0, 0.3, 0.05])
+    bnd = tlrgp.tile_bounds(n, 250)
+    for r in goldens["tlr_cholesky"][str(n)]["rules"]:
+        print(f"case N={n} b=250 {r['rule']}:{r['value']}", flush=True)
+        ll, fac, info = tlrgp.tlr_loglik(S, y, th, bnd, r["rule"], r["value"])
+        assert info["completed"] == r["completed"]
+        assert info["stored_sigma"] == r["stored_sigma"]
+        if r["completed"]:
+            assert info["stored_factor"] == r["stored_factor"]
+            assert info["max_rank_factor"] == r["max_rank_factor"]
+            assert abs(info["mean_rank_factor"] - r["mean_rank_factor"]) < 1e-12
+            assert _close(ll, r["loglik"]), (ll, r["loglik"])
+        else:
+            assert info["failed_at"] == r["failed_at_tile"]
+        fac = None
+    # release the large context (the facade caches up to four)
+    for c in list(tlrgp.likelihood._CTX_CACHE.values()):
+        c.close()
+    tlrgp.likelihood._CTX_CACHE.clear()

@@ -56,7 +56,7 @@ def test_morton_order_known_keys():
 
 
 def test_committed_fields_are_morton_ordered():
-    for n in (2000, 10000):
+    for n in (2000, 10000, 60000, 100000):
         S, _ = _golden_field(n)
         assert np.array_equal(O.morton_order(S), np.arange(n))
 
