@@ -22,7 +22,8 @@ namespace tlr {
 // ------------------------------------------------------------------------------------------ diagonal
 // D_i = sigma2 matern32(S_i, S_i) + sigma2 eta I, padded to b x b with the identity beyond the tile.
 __global__ void k_gen_diag(const double* __restrict__ sx, const double* __restrict__ sy, int64_t n, int b,
-                           MaternParams mp, double* __restrict__ D) {
+                           const MaternParams* __restrict__ mpp, double* __restrict__ D) {
+  const MaternParams mp = *mpp;
   const int i = blockIdx.y;
   const int64_t i0 = (int64_t)i * b;
   const int bi = (int)min<int64_t>(b, n - i0);
@@ -57,12 +58,14 @@ constexpr int MM_ROWS = 128, MM_COLS = 64, MM_QC = 64, MM_LD = MM_COLS + 1;
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256, 4) void k_matern_product(const double* __restrict__ sx, const double* __restrict__ sy,
-                                                           int64_t n, int b, MaternParams mp, int op,
+                                                           int64_t n, int b,
+                                                           const MaternParams* __restrict__ mpp, int op,
                                                            const double* __restrict__ X, int64_t xstride,
                                                            double* __restrict__ out, int64_t ostride, int k,
                                                            const int* __restrict__ active, int64_t t0) {
   const int64_t t = t0 + blockIdx.z;
   if (active && !active[t]) return;
+  const MaternParams mp = *mpp;
   int i, j;
   tri_decode(t, &i, &j);
   if (op) { const int x = i; i = j; j = x; }
@@ -158,12 +161,13 @@ __global__ __launch_bounds__(64 * NW) void k_panel_qr(double* __restrict__ M, in
 // partial sums over 64 x 64 blocks of ||T - A B^T||_F^2 and ||T||_F^2, written to
 // part[(t * nblk + blk) * 2 + {0, 1}].  Deterministic: the rank kernel adds them in block order.
 __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ sx, const double* __restrict__ sy,
-                                                  int64_t n, int b, MaternParams mp,
+                                                  int64_t n, int b, const MaternParams* __restrict__ mpp,
                                                   const double* __restrict__ A, const double* __restrict__ B,
                                                   int64_t stride, int k, double* __restrict__ part,
                                                   const int* __restrict__ active, int64_t t0) {
   const int64_t t = t0 + blockIdx.z;
   if (active && !active[t]) return;
+  const MaternParams mp = *mpp;
   int i, j;
   tri_decode(t, &i, &j);
   const int64_t r0g = (int64_t)i * b, q0g = (int64_t)j * b;
@@ -235,13 +239,15 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ sx,
 __global__ __launch_bounds__(256) void k_rank_decide(double* __restrict__ Rm, int64_t rstride,
                                                      const double* __restrict__ Qrow, const double* __restrict__ Qcol,
                                                      int64_t qstride, const double* __restrict__ part, int b, int k,
-                                                     int kmax, int pmin, int rule_fixed, double thr_rel, double thr_abs,
+                                                     int kmax, int pmin, int rule_fixed,
+                                                     const double* __restrict__ thr2,
                                                      double* __restrict__ U, double* __restrict__ V, int64_t fstride,
                                                      int rank_cap, int* __restrict__ rank, int* __restrict__ need,
                                                      int* __restrict__ retry_count, int* __restrict__ status,
                                                      const int* __restrict__ forced, int mixed) {
   const int64_t t = blockIdx.x;
   if (!need[t]) return;
+  const double thr_rel = thr2[0], thr_abs = thr2[1];
   extern __shared__ double lds[];
   double* G = lds;                // k x k
   double* J = G + k * k;          // k x k
@@ -424,14 +430,15 @@ hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t launch_gen_diag(const double* sx, const double* sy, int64_t n, int b, int nB, MaternParams mp,
+hipError_t launch_gen_diag(const double* sx, const double* sy, int64_t n, int b, int nB, const MaternParams* mp,
                            double* D, hipStream_t s) {
   dim3 grid(nblk((int64_t)b * b, 256 * 8), nB);
   hipLaunchKernelGGL(k_gen_diag, grid, dim3(256), 0, s, sx, sy, n, b, mp, D);
   return hipGetLastError();
 }
 
-hipError_t launch_matern_product(const double* sx, const double* sy, int64_t n, int b, int64_t ntile, MaternParams mp,
+hipError_t launch_matern_product(const double* sx, const double* sy, int64_t n, int b, int64_t ntile,
+                                 const MaternParams* mp,
                                  int op, const double* X, int64_t xstride, double* out, int64_t ostride, int k,
                                  const int* active, hipStream_t s) {
   for (int64_t t0 = 0; t0 < ntile; t0 += TILE_CHUNK) {
@@ -470,7 +477,7 @@ hipError_t launch_panel_qr(double* M, int64_t mstride, int b, int k, double* R, 
   return hipGetLastError();
 }
 
-hipError_t launch_residual(const double* sx, const double* sy, int64_t n, int b, int64_t ntile, MaternParams mp,
+hipError_t launch_residual(const double* sx, const double* sy, int64_t n, int b, int64_t ntile, const MaternParams* mp,
                            const double* A, const double* B, int64_t stride, int k, double* part, const int* active,
                            hipStream_t s) {
   const int nbb = nblk(b, 64);
@@ -488,7 +495,7 @@ size_t rank_decide_lds(int k) {
 
 hipError_t launch_rank_decide(double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
                               int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
-                              double thr_rel, double thr_abs, double* U, double* V, int64_t fstride, int rank_cap,
+                              const double* thr2, double* U, double* V, int64_t fstride, int rank_cap,
                               int* rank, int* need, int* retry_count, int* status, const int* forced, int mixed,
                               int64_t ntile, hipStream_t s) {
   const size_t lds = rank_decide_lds(k);
@@ -499,7 +506,7 @@ hipError_t launch_rank_decide(double* Rm, int64_t rstride, const double* Qrow, c
     attr_set = true;
   }
   hipLaunchKernelGGL(k_rank_decide, dim3((unsigned)ntile), dim3(256), lds, s, Rm, rstride, Qrow, Qcol, qstride, part,
-                     b, k, kmax, pmin, rule_fixed, thr_rel, thr_abs, U, V, fstride, rank_cap, rank, need, retry_count,
+                     b, k, kmax, pmin, rule_fixed, thr2, U, V, fstride, rank_cap, rank, need, retry_count,
                      status, forced, mixed);
   return hipGetLastError();
 }

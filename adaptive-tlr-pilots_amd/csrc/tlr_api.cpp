@@ -14,7 +14,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/tlrgp.h"
@@ -90,6 +92,12 @@ struct tlr_ctx {
   std::vector<void*> allocs;
   std::vector<Slot> slots;
   std::vector<SlotState> sstate;       // per slot: the factor it holds
+  // captured evaluations (tlr_loglik_batch_graph), by (m, rule, precision); dropped when a tunable changes
+  std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
+  void drop_graphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
   int64_t factorizations = 0;          // evaluations run (theta factorized), for reports
   SlotPtrs* d_slots = nullptr;
   int64_t scr_stride = 0;
@@ -125,6 +133,9 @@ struct tlr_ctx {
   int* rep_rpl = nullptr;              // max_batch records of recompression arithmetic (nlog each)
   int* d_retry = nullptr;              // per slot: tiles redone with the wider sketch
   double* d_thr = nullptr;             // per slot (thr_rel, thr_abs) of the current batch
+  MaternParams* d_mp = nullptr;        // per slot Matern parameters of the current batch (read by the kernels)
+  MaternParams* h_mp = nullptr;        // pinned staging of d_mp
+  std::vector<MaternParams> h_mp_host;
   double* h_thr = nullptr;             // pinned staging of d_thr
   int* d_replay = nullptr;             // per slot: record whose ranks are replayed, or -1
   int* h_replay = nullptr;             // pinned staging of d_replay
@@ -148,9 +159,11 @@ struct tlr_ctx {
   double timed_ms = 0.0, timed_flops = 0.0, timed_bytes = 0.0;
   int64_t timed_launches = 0;
   ~tlr_ctx() {
+    drop_graphs();
     if (pred_buf) (void)hipFree(pred_buf);
     if (stash) (void)hipFree(stash);
     if (h_thr && !arena) (void)hipHostFree(h_thr);
+    if (h_mp && !arena) (void)hipHostFree(h_mp);
     if (h_replay && !arena) (void)hipHostFree(h_replay);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
@@ -421,14 +434,18 @@ static int ctx_build(tlr_ctx* c, const double* S, const double* y, int64_t n, in
   ACHK(c->alloc(&c->d_slots, (size_t)max_batch));
   ACHK(c->alloc(&c->d_retry, (size_t)max_batch));
   ACHK(c->alloc(&c->d_thr, (size_t)2 * max_batch));
+  ACHK(c->alloc(&c->d_mp, (size_t)max_batch));
   if (c->arena) {   // pageable staging (the copies on the null stream complete before they return)
     c->h_replay_host.assign(max_batch, -1);
     c->h_thr_host.assign(2 * (size_t)max_batch, 0.0);
+    c->h_mp_host.assign(max_batch, MaternParams{0.0, 0.0, 0.0});
     c->h_replay = c->h_replay_host.data();
     c->h_thr = c->h_thr_host.data();
+    c->h_mp = c->h_mp_host.data();
   } else {
     ACHK(hipHostMalloc((void**)&c->h_replay, sizeof(int) * max_batch, hipHostMallocDefault));
     ACHK(hipHostMalloc((void**)&c->h_thr, sizeof(double) * 2 * max_batch, hipHostMallocDefault));
+    ACHK(hipHostMalloc((void**)&c->h_mp, sizeof(MaternParams) * max_batch, hipHostMallocDefault));
   }
   std::vector<SlotPtrs> hp(max_batch);
   for (int s = 0; s < max_batch; ++s) hp[s] = c->slots[s].p;
@@ -474,8 +491,10 @@ int tlr_ctx_set_y(tlr_ctx* ctx, const double* y) {
 // stash record `warm` (the recorded evaluation at a nearby theta), and two products (T X, T^T Q) replace the
 // four of a cold start.
 // cs: the compression stream (and workspace set) this slot uses.
-static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternParams& mp, int rule_fixed,
-                         double thr_rel, double thr_abs, const int* forced, int mixed, int warm, int cs) {
+static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, int rule_fixed, const int* forced, int mixed,
+                         int warm, int cs) {
+  const MaternParams* mp = c->d_mp + slot;
+  const double* thr2 = c->d_thr + 2 * slot;
   hipStream_t s = c->cls_stream[cs];
   double* const bufA = c->bufA[cs];
   double* const bufB = c->bufB[cs];
@@ -545,7 +564,7 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, const MaternPa
     c->toc(on, tq);
     const double* Qrow = (last_op == 0) ? M : Qprev;
     const double* Qcol = (last_op == 0) ? Qprev : M;
-    HIPCHK(launch_rank_decide(bufR, rs, Qrow, Qcol, ms, part, b, k, kmax, PMIN, rule_fixed, thr_rel, thr_abs,
+    HIPCHK(launch_rank_decide(bufR, rs, Qrow, Qcol, ms, part, b, k, kmax, PMIN, rule_fixed, thr2,
                               p.U, p.V, (int64_t)b * RANK_CAP, RANK_CAP, p.rank, need, c->d_retry + slot,
                               p.status, forced, mixed, nt, s));
   }
@@ -704,9 +723,13 @@ static int stash_compressed(tlr_ctx* c, int m) {
 // reuse[s] >= 0 makes it also take record reuse[s]'s stashed compressed factors instead of compressing
 // (valid when theta differs from the record's only in eta, which leaves the off-diagonal tiles unchanged);
 // stash != 0 stores every slot's compressed factors as records 0..m-1.
+// graph: 0 enqueue and run; 1 capture the device work into *exec (nothing runs, no results); 2 run *exec (a
+// capture of the same configuration) instead of enqueueing.  The host staging of theta (thresholds, Matern
+// parameters, replay records) happens in every mode: a captured graph copies it from the pinned buffers.
 static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value,
                           int32_t precision, const int* replay, double* out, int32_t* status, int32_t* failed_at,
-                          int32_t* ranks_sigma, int32_t* ranks_factor, const int* reuse, int stash, const int* warm) {
+                          int32_t* ranks_sigma, int32_t* ranks_factor, const int* reuse, int stash, const int* warm,
+                          int graph = 0, hipGraphExec_t* exec = nullptr) {
   if (!c || !thetas || !out || m < 1 || m > c->max_batch) return set_err(TLR_ERR_ARG, "invalid argument");
   if (rule != TLR_RULE_FIXED && rule != TLR_RULE_ADAPTIVE && rule != TLR_RULE_MIXED)
     return set_err(TLR_ERR_ARG, "unknown rule");
@@ -723,26 +746,32 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
   hipStream_t st = c->stream;
   const int rule_fixed = rule == TLR_RULE_FIXED;
   const int mixed = rule == TLR_RULE_MIXED;
-  for (int s = 0; s < m; ++s) c->sstate[s].valid = false;
-  c->factorizations += m;
+  if (graph != 1) {   // a capture runs nothing: the slots keep their factors
+    for (int s = 0; s < m; ++s) c->sstate[s].valid = false;
+    c->factorizations += m;
+  }
   c->ev_used = 0;
   c->timed.clear();
-  HIPCHK(launch_fill_int(c->d_retry, c->max_batch, 0, st));
-  HIPCHK(launch_fill_int(c->wl_count, (int64_t)WL_STRIDE * RCLASSES * c->nB, 0, st));
-  HIPCHK(launch_fill_int(c->wlp_count, (int64_t)WL_STRIDE * RCLASSES * c->nB, 0, st));
-  HIPCHK(launch_fill_int(c->wlr_count, (int64_t)WL_STRIDE * 4 * c->nB, 0, st));
-  // per-slot thresholds of the recompression (the adaptive budget depends on theta); the previous call
-  // ended with a synchronisation, so the pinned staging buffer is free
+  // per-slot thresholds of the recompression (the adaptive budget depends on theta), Matern parameters and
+  // replay records; the previous call ended with a synchronisation, so the pinned staging buffers are free
   const double bmax_ = (double)(c->n < c->b ? c->n : c->b);
   for (int s = 0; s < m; ++s) {
     const double* th = thetas + 3 * s;
     c->h_thr[2 * s] = rule == TLR_RULE_FIXED ? value : 0.0;
     c->h_thr[2 * s + 1] = rule == TLR_RULE_FIXED ? 0.0 : value * bmax_ * th[0] * th[2] / (double)c->n;
+    c->h_mp[s].sigma2 = th[0];
+    c->h_mp[s].inv_range_sqrt3 = sqrt(3.0) / th[1];
+    c->h_mp[s].nugget = th[0] * th[2];
+    c->h_replay[s] = replay ? replay[s] : -1;
   }
+  auto enqueue = [&]() -> int {
+  HIPCHK(launch_fill_int(c->d_retry, c->max_batch, 0, st));
+  HIPCHK(launch_fill_int(c->wl_count, (int64_t)WL_STRIDE * RCLASSES * c->nB, 0, st));
+  HIPCHK(launch_fill_int(c->wlp_count, (int64_t)WL_STRIDE * RCLASSES * c->nB, 0, st));
+  HIPCHK(launch_fill_int(c->wlr_count, (int64_t)WL_STRIDE * 4 * c->nB, 0, st));
   HIPCHK(hipMemcpyAsync(c->d_thr, c->h_thr, sizeof(double) * 2 * m, hipMemcpyHostToDevice, st));
-  for (int s = 0; s < m; ++s) c->h_replay[s] = replay ? replay[s] : -1;
+  HIPCHK(hipMemcpyAsync(c->d_mp, c->h_mp, sizeof(MaternParams) * m, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(c->d_replay, c->h_replay, sizeof(int) * m, hipMemcpyHostToDevice, st));
-  const double bmax = (double)(c->n < c->b ? c->n : c->b);
   // per-slot state on the main stream first; then the slots are compressed in turn on the main stream and
   // the two side streams (each with its own workspace), which overlap one slot's latency-bound kernels
   // (panel QR, rank decision) and launch tails with another's.  When a compression kernel is being timed
@@ -763,22 +792,15 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
   }
   int ncomp = 0;   // slots compressed so far (stream of the next: ncomp % ncs)
   for (int s = 0; s < m; ++s) {
-    const double* th = thetas + 3 * s;
     const SlotPtrs& p = c->slots[s].p;
-    MaternParams mp;
-    mp.sigma2 = th[0];
-    mp.inv_range_sqrt3 = sqrt(3.0) / th[1];
-    mp.nugget = th[0] * th[2];
-    const double thr_abs = rule_fixed ? 0.0 : value * bmax * th[0] * th[2] / (double)c->n;
     const int* forced = (replay && replay[s] >= 0) ? c->rep_rank0 + (int64_t)replay[s] * c->ntile : nullptr;
     if (reuse && reuse[s] >= 0) {
       const int rec = reuse[s];
-      HIPCHK(launch_gen_diag(c->sx, c->sy, c->n, c->b, c->nB, mp, p.D, st));
+      HIPCHK(launch_gen_diag(c->sx, c->sy, c->n, c->b, c->nB, c->d_mp + s, p.D, st));
       HIPCHK(launch_stash(p.U, p.V, (int64_t)c->b * RANK_CAP, c->b, c->rep_rank0 + (int64_t)rec * c->ntile,
                           c->d_stash_off + (int64_t)rec * c->ntile, c->stash, p.rank, p.rank0, c->ntile, 1, st));
     } else {
-      int rc = compress_slot(c, s, p, mp, rule_fixed, value, thr_abs, forced, mixed, warm ? warm[s] : -1,
-                             ncomp % ncs);
+      int rc = compress_slot(c, s, p, rule_fixed, forced, mixed, warm ? warm[s] : -1, ncomp % ncs);
       if (rc != TLR_OK) return rc;
       ++ncomp;
     }
@@ -811,6 +833,28 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
     }
   }
   HIPCHK(launch_final(sa, m, st));
+  return TLR_OK;
+  };   // enqueue
+  if (graph == 2) {
+    HIPCHK(hipGraphLaunch(*exec, st));
+  } else if (graph == 1) {
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc != TLR_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return set_err(TLR_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return set_err(TLR_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    return TLR_OK;
+  } else {
+    const int rc = enqueue();
+    if (rc != TLR_OK) return rc;
+  }
   HIPCHK(hipStreamSynchronize(st));
   if (!c->timed.empty()) {
     std::vector<int> retry(c->max_batch);
@@ -890,9 +934,10 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
 // call finds no work in flight (it reuses the pinned staging buffers and the work lists).
 static int run_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
                      const int* replay, double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma,
-                     int32_t* ranks_factor, const int* reuse = nullptr, int stash = 0, const int* warm = nullptr) {
+                     int32_t* ranks_factor, const int* reuse = nullptr, int stash = 0, const int* warm = nullptr,
+                     int graph = 0, hipGraphExec_t* exec = nullptr) {
   const int rc = run_batch_impl(c, m, thetas, rule, value, precision, replay, out, status, failed_at, ranks_sigma,
-                                ranks_factor, reuse, stash, warm);
+                                ranks_factor, reuse, stash, warm, graph, exec);
   if (rc < 0 && c) {
     for (int q = 1; q < RCLASSES; ++q)
       if (c->cls_stream[q]) (void)hipStreamSynchronize(c->cls_stream[q]);
@@ -957,6 +1002,7 @@ int tlr_loglik_in_arena(void* base, size_t bytes, int grid_cap, const double* S,
 
 int tlr_ctx_set_tunable(tlr_ctx* ctx, int32_t which, double value) {
   if (!ctx) return set_err(TLR_ERR_ARG, "null context");
+  ctx->drop_graphs();   // the tunables are baked into captured launches
   switch (which) {
     case TLR_TUNE_GUARD:
       if (!(value >= 1.0 && value <= 1e12)) return set_err(TLR_ERR_ARG, "guard must be in [1, 1e12]");
@@ -1008,6 +1054,30 @@ extern "C" {
 int tlr_loglik_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
                      double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma, int32_t* ranks_factor) {
   return run_batch(c, m, thetas, rule, value, precision, nullptr, out, status, failed_at, ranks_sigma, ranks_factor);
+}
+
+int tlr_loglik_batch_graph(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value,
+                           int32_t precision, double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma,
+                           int32_t* ranks_factor) {
+  if (!c) return set_err(TLR_ERR_ARG, "null context");
+  if (c->single_stream) return set_err(TLR_ERR_ARG, "graph capture needs a context with its own streams");
+  if (c->timing_kernel != 0) return set_err(TLR_ERR_ARG, "graph capture with kernel timing on");
+  const auto key = std::make_tuple((int)m, (int)rule, (int)precision);
+  auto it = c->graphs.find(key);
+  if (it != c->graphs.end())
+    return run_batch(c, m, thetas, rule, value, precision, nullptr, out, status, failed_at, ranks_sigma, ranks_factor,
+                     nullptr, 0, nullptr, 2, &it->second);
+  // first call of this configuration: run it as usual (every kernel attribute is set and every code object
+  // loaded before a capture), then capture the same device work for the next calls
+  const int rc = run_batch(c, m, thetas, rule, value, precision, nullptr, out, status, failed_at, ranks_sigma,
+                           ranks_factor);
+  if (rc < 0) return rc;
+  hipGraphExec_t exec = nullptr;
+  const int rc2 = run_batch(c, m, thetas, rule, value, precision, nullptr, out, status, failed_at, nullptr, nullptr,
+                            nullptr, 0, nullptr, 1, &exec);
+  if (rc2 < 0) return rc2;
+  c->graphs[key] = exec;
+  return rc;
 }
 
 int tlr_loglik_grad_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,

@@ -84,20 +84,23 @@ struct SlotArgs {
 };
 
 // compress.hip
-hipError_t launch_gen_diag(const double* sx, const double* sy, int64_t n, int b, int nB, MaternParams mp, double* D,
-                           hipStream_t s);
-hipError_t launch_matern_product(const double* sx, const double* sy, int64_t n, int b, int64_t ntile, MaternParams mp,
+// mp: device pointer to the slot's Matern parameters (read by the kernels, so that a captured launch sees the
+// current theta); thr2: device pointer to the slot's (thr_rel, thr_abs)
+hipError_t launch_gen_diag(const double* sx, const double* sy, int64_t n, int b, int nB, const MaternParams* mp,
+                           double* D, hipStream_t s);
+hipError_t launch_matern_product(const double* sx, const double* sy, int64_t n, int b, int64_t ntile,
+                                 const MaternParams* mp,
                                  int op, const double* X, int64_t xstride, double* out, int64_t ostride, int k,
                                  const int* active, hipStream_t s);
 hipError_t launch_panel_qr(double* M, int64_t mstride, int b, int k, double* R, int64_t rstride, int r_transposed,
                            const int* active, int64_t ntile, hipStream_t s);
-hipError_t launch_residual(const double* sx, const double* sy, int64_t n, int b, int64_t ntile, MaternParams mp,
+hipError_t launch_residual(const double* sx, const double* sy, int64_t n, int b, int64_t ntile, const MaternParams* mp,
                            const double* A, const double* B, int64_t stride, int k, double* part, const int* active,
                            hipStream_t s);
 size_t rank_decide_lds(int k);
 hipError_t launch_rank_decide(double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
                               int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
-                              double thr_rel, double thr_abs, double* U, double* V, int64_t fstride, int rank_cap,
+                              const double* thr2, double* U, double* V, int64_t fstride, int rank_cap,
                               int* rank, int* need, int* retry_count, int* status, const int* forced, int mixed,
                               int64_t ntile, hipStream_t s);
 hipError_t launch_warm_sketch(const double* stash, const int64_t* off, const int* r0, const double* omega, double* X,

@@ -29,6 +29,8 @@ SIGNATURES = {
     "tlr_loglik": (ctypes.c_int, [_VP, _D, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _D, _I, _I, _I]),
     "tlr_loglik_batch": (ctypes.c_int, [_VP, ctypes.c_int32, _D, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
                                         _D, _I, _I, _I, _I]),
+    "tlr_loglik_batch_graph": (ctypes.c_int, [_VP, ctypes.c_int32, _D, ctypes.c_int32, ctypes.c_double,
+                                              ctypes.c_int32, _D, _I, _I, _I, _I]),
     "tlr_loglik_grad_batch": (ctypes.c_int, [_VP, ctypes.c_int32, _D, ctypes.c_int32, ctypes.c_double,
                                              ctypes.c_int32, ctypes.c_double, _D, _D, _I, _I, _I, _I]),
     "tlr_loglik_grad": (ctypes.c_int, [_VP, _D, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_double,

@@ -153,9 +153,10 @@ class TLRContext:
     def set_y(self, y):
         check(lib.tlr_ctx_set_y(self._ptr, dptr(f64(y, (self.n,)))))
 
-    def loglik_batch(self, thetas, rule, value, precision="fp64", ranks=True):
+    def loglik_batch(self, thetas, rule, value, precision="fp64", ranks=True, graph=False):
         """Evaluate m <= max_batch theta in one pass.  Returns dict of arrays: loglik, logdet, quad (NaN when
-        not PD), completed (bool), failed_at (-1 when completed), ranks_sigma / ranks_factor (m, nB, nB)."""
+        not PD), completed (bool), failed_at (-1 when completed), ranks_sigma / ranks_factor (m, nB, nB).
+        graph: replay a HIP graph of this configuration (captured on its first use; same results)."""
         th = f64(np.atleast_2d(thetas))
         m = th.shape[0]
         if th.shape[1] != 3 or m > self.max_batch:
@@ -165,8 +166,9 @@ class TLRContext:
         fa = np.empty(m, np.int32)
         rs = np.zeros((m, self.nB, self.nB), np.int32) if ranks else None
         rf = np.zeros((m, self.nB, self.nB), np.int32) if ranks else None
-        check(lib.tlr_loglik_batch(self._ptr, m, dptr(th), RULES[rule], float(value), PRECISIONS[precision],
-                                   dptr(out), iptr(st), iptr(fa), iptr(rs), iptr(rf)))
+        fn = lib.tlr_loglik_batch_graph if graph else lib.tlr_loglik_batch
+        check(fn(self._ptr, m, dptr(th), RULES[rule], float(value), PRECISIONS[precision], dptr(out), iptr(st),
+                 iptr(fa), iptr(rs), iptr(rf)))
         res = dict(loglik=out[:, 0], logdet=out[:, 1], quad=out[:, 2], completed=st == _lib.TLR_OK,
                    failed_at=fa, ranks_sigma=rs, ranks_factor=rf)
         if rule == "mixed":

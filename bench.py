@@ -36,10 +36,11 @@ sys.path.insert(0, os.path.join(ROOT, "adaptive-tlr-pilots_amd"))
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 157.3}   # MI355X dense FP64 / FP32 (AMD published)
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per factorization step and theta (FETCH_SIZE + WRITE_SIZE of the step's dispatches, each counter
-# calibrated against a device copy of known size, divided by the theta of the batch); measured with
-# rocprofv3 --pmc on this workload in plain FP64 (profiles/recompress_n40000_b500_pmc_traffic.md).
-STEP_TRAFFIC_PER_THETA = 5.63e9 / 20
+# HBM bytes per factorization step and theta (FETCH_SIZE + WRITE_SIZE of the factorization's dispatches, each
+# counter calibrated against a device copy of known size, divided by steps and theta); measured with
+# rocprofv3 --pmc on this workload in plain FP64
+# (profiles/factorization_and_compression_n40000_b500_pmc_traffic.md).
+STEP_TRAFFIC_PER_THETA = 0.395e9
 # central-difference step of the gradient in log theta: 1e-5 in FP64; the reduced-precision modes take a
 # larger step so that their rounding (not smooth in theta) stays small against the difference
 GRAD_H = {"fp64": 1e-5, "fp32": 1e-3, "bf16": 1e-3}

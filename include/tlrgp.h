@@ -89,6 +89,17 @@ int tlr_loglik(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, 
 int tlr_loglik_batch(tlr_ctx* ctx, int32_t m, const double* thetas, int32_t rule, double value, int32_t precision,
                      double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma, int32_t* ranks_factor);
 
+/* tlr_loglik_batch through a HIP graph: the first call of a configuration (m, rule, precision) runs as
+ * tlr_loglik_batch and then captures the same device work (compression, every factorization step on the
+ * context's streams, the reductions) into a graph; later calls with that configuration copy the new theta and
+ * rule value into the context's staging buffers and launch the graph, one launch per batch instead of about
+ * twenty per tile step.  Results equal tlr_loglik_batch's bit for bit.  Changing a tunable drops the graphs.
+ * No counterpart in the reference (the batched sampler evaluation of 38:89-126, 19:170-203 is one process per
+ * chain). */
+int tlr_loglik_batch_graph(tlr_ctx* ctx, int32_t m, const double* thetas, int32_t rule, double value,
+                           int32_t precision, double* out, int32_t* status, int32_t* failed_at, int32_t* ranks_sigma,
+                           int32_t* ranks_factor);
+
 /* Log-likelihood and its gradient in (log sigma2, log range, log eta) at frozen ranks: the ranks of every
  * tile after the initial compression and after every recompression are those chosen at theta.  The
  * sigma2 component is exact (0.5 * quad - N / 2: tiles and budgets scale together with sigma2); the range
