@@ -293,6 +293,23 @@ def batched_loglik_grad(ctx, rule, value, h=1e-5, precision="fp64"):
     return f
 
 
+def concurrent_loglik_grad(parts):
+    """One evaluator over chains on several fields: parts is a list of (loglik_grad, k) with each field's
+    batched evaluator (batched_loglik_grad on its own TLRContext) and its number of chains; the returned
+    function takes the (sum k, 3) log theta of all chains, evaluates every field's share at the same time (one
+    host thread per field; each context has its own streams and the library releases the interpreter during
+    its calls) and returns the concatenated (log L, gradient)."""
+    from concurrent.futures import ThreadPoolExecutor
+    offs = np.cumsum([0] + [k for _, k in parts])
+    pool = ThreadPoolExecutor(max_workers=len(parts))
+
+    def f(log_thetas):
+        futs = [pool.submit(fn, log_thetas[offs[i]:offs[i + 1]]) for i, (fn, _) in enumerate(parts)]
+        res = [fu.result() for fu in futs]
+        return np.concatenate([r[0] for r in res]), np.concatenate([r[1] for r in res])
+    return f
+
+
 def run_chains_batched(lik_batch, starts, seeds, iters, burn, ckpt=None, ckpt_every=100):
     """M random-walk MH chains advanced in lockstep, the proposals of one iteration evaluated as one batch.
     Chain c follows exactly the steps of run_chain(start=starts[c], seed=seeds[c]): its own generator,

@@ -177,3 +177,22 @@ def test_split_rhat_ess():
     assert rh2 > 1.2
     s = I.pool_chains(0.1 * _ar1(0.3, 3000, 4, m=9).reshape(3, 3, 3000).transpose(0, 2, 1), burn=500)
     assert s["n_chains"] == 3 and len(s["split_rhat"]) == 3 and len(s["ci95"]) == 3
+
+
+def test_concurrent_loglik_grad_splits_and_joins():
+    """The multi-field evaluator hands each field its own rows, in order, and concatenates the results."""
+    from tlrgp.inference import concurrent_loglik_grad
+    seen = []
+
+    def make(tag):
+        def fn(x):
+            seen.append((tag, x.shape[0]))
+            return x.sum(1) + tag, x * tag
+        return fn
+
+    f = concurrent_loglik_grad([(make(1.0), 3), (make(10.0), 2)])
+    x = np.arange(15, dtype=float).reshape(5, 3)
+    ll, g = f(x)
+    np.testing.assert_array_equal(ll, np.r_[x[:3].sum(1) + 1.0, x[3:].sum(1) + 10.0])
+    np.testing.assert_array_equal(g, np.r_[x[:3] * 1.0, x[3:] * 10.0])
+    assert sorted(seen) == [(1.0, 3), (10.0, 2)]
