@@ -84,11 +84,18 @@ def cpu_baseline(S, y, b, theta, rule, value, n_sample, evals_per_point):
     from threadpoolctl import threadpool_limits
     cores = os.cpu_count()
     n_full = len(y)
+    t_start = time.perf_counter()
+
+    def progress(phase, done, total):   # the oracle's own progress, every tenth tile row or step
+        if done % 10 == 0 or done == total:
+            print(f"# cpu baseline: {phase} {done}/{total} ({time.perf_counter() - t_start:.0f} s)", file=sys.stderr,
+                  flush=True)
+
     with threadpool_limits(limits=cores):
         Ss, ys = S[:n_sample], y[:n_sample]
         bnd = O.tile_bounds(n_sample, b)
         t0 = time.perf_counter()
-        ll, _, info = O.tlr_loglik(Ss, ys, theta, bnd, rule, value)
+        ll, _, info = O.tlr_loglik(Ss, ys, theta, bnd, rule, value, progress=progress)
         t = time.perf_counter() - t0
     th = [float(x) for x in theta]
     if n_sample >= n_full:

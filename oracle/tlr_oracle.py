@@ -322,14 +322,15 @@ def round_pair(U, V, budget):
 
 
 # ----------------------------------------------------------------------------- TLR Cholesky
-def tlr_cholesky(S, theta, bounds, rule, value, frozen=None):
+def tlr_cholesky(S, theta, bounds, rule, value, frozen=None, progress=None):
     """Right-looking TLR Cholesky of Sigma(theta) without forming it.  Follows 17:137-215.
 
     Returns (L_diag, L_off, info); (None, None, info) with completed=False, failed_at=k when the
     diagonal tile k is not positive definite.  info adds 'ranks_sigma' and 'ranks_factor' (nB x nB
     int arrays, lower triangle) to the reference's keys, and 'rank_log' ({(k, i, j): rank} of every
     recompression).  frozen: the info of an earlier evaluation whose ranks_sigma and rank_log are replayed
-    instead of the rule's choices (frozen ranks, for derivatives; not in the reference)."""
+    instead of the rule's choices (frozen ranks, for derivatives; not in the reference).  progress: called
+    as progress(phase, done, total) after every tile row of the compression and every factorization step."""
     n = S.shape[0]
     nB = len(bounds)
     b = max(e - s for s, e in bounds)
@@ -357,6 +358,8 @@ def tlr_cholesky(S, theta, bounds, rule, value, frozen=None):
             A_off[(i, j)] = (U, V)
             ranks0[i, j] = U.shape[1]
             stored0 += 2 * U.shape[1] * max(i1 - i0, j1 - j0)
+        if progress:
+            progress("compression", i + 1, nB)
     rank_log = {}
     info = dict(stored_sigma=int(stored0), max_rank_sigma=int(ranks0.max()) if nB > 1 else 0,
                 ranks_sigma=ranks0, rank_log=rank_log)
@@ -397,6 +400,8 @@ def tlr_cholesky(S, theta, bounds, rule, value, frozen=None):
                     if mixed:
                         Ur, Vr, _ = round_pair(Ur, Vr, thr_abs - tail)
                     A_off[(i, j)] = (Ur, Vr)
+        if progress:
+            progress("factorization", k + 1, nB)
     ranksF = np.zeros((nB, nB), np.int64)
     for (i, j), (U, _) in L_off.items():
         ranksF[i, j] = U.shape[1]
@@ -475,10 +480,10 @@ def tlr_matvec_L(L_diag, L_off, bounds, Z):
     return Y
 
 
-def tlr_loglik(S, y, theta, bounds, rule, value, frozen=None):
+def tlr_loglik(S, y, theta, bounds, rule, value, frozen=None, progress=None):
     """TLR log-likelihood; (ll, (L_diag, L_off), info) or (None, None, info).  Follows 17:259-266.
-    info additionally carries 'logdet' and 'quad' (z^T z); frozen as in tlr_cholesky."""
-    L_diag, L_off, info = tlr_cholesky(S, theta, bounds, rule, value, frozen)
+    info additionally carries 'logdet' and 'quad' (z^T z); frozen and progress as in tlr_cholesky."""
+    L_diag, L_off, info = tlr_cholesky(S, theta, bounds, rule, value, frozen, progress)
     if L_diag is None:
         return None, None, info
     z = tlr_forward(L_diag, L_off, bounds, y[:, None])[:, 0]
