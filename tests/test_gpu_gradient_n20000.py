@@ -48,7 +48,9 @@ def test_gradient_six_fields(tl, fx):
         assert abs(ll - fx["ll_tlr"][f]) <= 1e-6 * max(1.0, abs(fx["ll_tlr"][f])), (ll, fx["ll_tlr"][f])
         # frozen-rank central differences of the oracle: the same derivative
         np.testing.assert_allclose(g, g_fd, rtol=1e-5, atol=1e-5 * np.abs(g_fd).max())
-        rel = np.abs(g - g_d) / np.maximum(1.0, np.abs(g_d))
+        # relative to the gradient's largest component: a component near zero (eta at the true theta) is a
+        # small difference of large terms, and the rule's approximation moves it by a share of those terms
+        rel = np.abs(g - g_d) / max(1.0, float(np.abs(g_d).max()))
         worst_dense = max(worst_dense, float(rel.max()))
         # reduced-precision arithmetic (frozen ranks and arithmetic, step 1e-3) against FP64
         r32 = ctx.loglik_grad_batch(th[None, :], "adaptive", 0.9, h=1e-3, precision="fp32", ranks=False)
