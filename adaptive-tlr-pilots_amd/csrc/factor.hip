@@ -581,6 +581,20 @@ __device__ __forceinline__ void push_redo(const SlotArgs& sa, int k, int panel, 
   else atomicMax(&sa.slot(slot).status[ST_GUARD], 3);
 }
 
+// Block Gram-Schmidt of the panel against the r1 columns of A (global, or the LDS copy), applied twice:
+// coefficients (scaled by scl when given) to R (leading dimension ldr), their sum over both passes.
+template <int RPL, int NW, int CPW, class T, class S>
+__device__ __forceinline__ void cgs2(RegPanel<RPL, NW, CPW, T>& pan, const S* A, int64_t lda, int m, int r1, int r2,
+                                     const T* scl, T* R, int ldr, T* Ct) {
+  panel_coef(pan, A, lda, m, r1, r2, scl, R, ldr);
+  __syncthreads();
+  panel_axpy(pan, A, lda, m, r1, r2, scl, R, ldr, (T)-1);
+  panel_coef(pan, A, lda, m, r1, r2, scl, Ct, r1);
+  __syncthreads();
+  panel_axpy(pan, A, lda, m, r1, r2, scl, Ct, r1, (T)-1);
+  for (int e = threadIdx.x; e < r1 * r2; e += 64 * NW) R[(e % r1) + (e / r1) * ldr] += Ct[e];
+}
+
 // One pair (i, j) = pair index q of step k, for slot `slot`; scr is the work group's scratch.  PM is the
 // arithmetic (working type T: double for PM_FP64, float otherwise; PM_BF16 rounds the operands of the two
 // products W = V_ik^T V_jk and U_ik W to bfloat16).  panel: the pair came from a panel list (j = k + 1).
@@ -656,15 +670,7 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
 #pragma unroll
       for (int t = 0; t < RPL; ++t) nu2 = fma(pan.a[jj][t], pan.a[jj][t], nu2);
   }
-  if (r1 > 0) {
-    panel_coef(pan, Vij, b, b, r1, r2, (const T*)nullptr, RvR, w);
-    __syncthreads();
-    panel_axpy(pan, Vij, b, b, r1, r2, (const T*)nullptr, RvR, w, (T)-1);
-    panel_coef(pan, Vij, b, b, r1, r2, (const T*)nullptr, Ct, r1);
-    __syncthreads();
-    panel_axpy(pan, Vij, b, b, r1, r2, (const T*)nullptr, Ct, r1, (T)-1);
-    for (int e = tid; e < r1 * r2; e += NT) RvR[(e % r1) + (e / r1) * w] += Ct[e];
-  }
+  if (r1 > 0) cgs2(pan, Vij, b, b, r1, r2, (const T*)nullptr, RvR, w, Ct);
   pan.geqr2(b, r2, vbuf, tau);
   pan.store_r(RvR + r1, w, r2);            // R2v in rows r1.. of the right block
   pan.org2r(b, r2, vbuf, tau);
@@ -685,18 +691,17 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
 #pragma unroll
       for (int t = 0; t < RPL; ++t) nx2 = fma(pan.a[jj][t], pan.a[jj][t], nx2);
   }
-  if (r1 > 0) {
-    panel_coef(pan, Uij, b, b, r1, r2, sc, RuR, w);
-    __syncthreads();
-    panel_axpy(pan, Uij, b, b, r1, r2, sc, RuR, w, (T)-1);
-    panel_coef(pan, Uij, b, b, r1, r2, sc, Ct, r1);
-    __syncthreads();
-    panel_axpy(pan, Uij, b, b, r1, r2, sc, Ct, r1, (T)-1);
-    for (int e = tid; e < r1 * r2; e += NT) RuR[(e % r1) + (e / r1) * w] += Ct[e];
-  }
+  if (r1 > 0) cgs2(pan, Uij, b, b, r1, r2, (const T*)sc, RuR, w, Ct);
   pan.geqr2(b, r2, vbuf, tau);
   pan.store_r(RuR + r1, w, r2);
   pan.org2r(b, r2, vbuf, tau);   // Q2u in registers from here to the outputs (ends with a barrier)
+  // narrow cores in the 4-wave classes form their outputs row by row (phase 5), reading Q2u back from the
+  // work group's scratch past Q2v (the region the 8-wave classes use for R_u, R_v; these classes keep
+  // those in LDS)
+  constexpr int XO = 16;
+  const bool rowout = NW == 4 && w <= XO;
+  T* Q2u = scrT + (int64_t)b * 64;
+  if (rowout) pan.store(Q2u, b, b, r2);
   // ---- 4. core K = R_u R_v^T, stored transposed: the one-sided Jacobi runs on K^T (on these cores it
   // needs fewer sweeps than on K), giving K^T J = V S (columns orthogonal, norms S) and J = U; truncation
   T nk2 = (T)0;
@@ -763,14 +768,70 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
   }
   __syncthreads();
   const int rn = s_rank;
-  // ---- 5. outputs, row blocks of RB rows staged through LDS (in place); register row block t of the
-  // panel covers rows [64 t, 64 t + 64), so t stays a compile-time index
-  // kept column c: U-side coefficients J[:, c] s_c, V-side (K^T J)[:, c] / s_c (orthonormal), scaled as used
+  // ---- 5. outputs
+  // kept column c: U-side coefficients J[:, c] s_c, V-side (K^T J)[:, c] / s_c (orthonormal), scaled as used.
+  // Narrow cores (rowout): each thread forms whole rows, reading the row's w inputs once (every load of a
+  // row in flight together) and writing its rn outputs in place, with no barrier between row blocks; the
+  // sums run in the same order as in the staged form below, so the values are the same.
+  if (rowout) {
+    // rows p = tid + NT h (h < NR): the inputs of all of a thread's rows are loaded before the first sum
+    // (one round of loads per side)
+    constexpr int NR = (64 * RPL + NT - 1) / NT;
+    T x[NR][XO];
+    // V_ij <- [V_ij, Q2v] V_core[:, perm]
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+      const int p = tid + NT * h;
+#pragma unroll
+      for (int m = 0; m < XO; ++m)
+        x[h][m] = !(p < b && m < w) ? (T)0 : (m < r1 ? (T)Vij[p + (int64_t)m * b] : Q2v[p + (int64_t)(m - r1) * b]);
+    }
+    for (int c = 0; c < rn; ++c) {
+      const int pc = perm[c];
+      const T* Vc = G + pc * w;
+      const T sv = sig[pc];
+#pragma unroll
+      for (int h = 0; h < NR; ++h) {
+        const int p = tid + NT * h;
+        T s = (T)0;
+#pragma unroll
+        for (int m = 0; m < XO; ++m)
+          if (m < w) s = fma(x[h][m], Vc[m], s);
+        if (p < b) Vij[p + (int64_t)c * b] = sv > (T)0 ? (double)(s / sv) : 0.0;
+      }
+    }
+    // U_ij <- [Q1, Q2u] (U S)_core[:, perm],  Q1 = U_ij diag(1/sigma)
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+      const int p = tid + NT * h;
+#pragma unroll
+      for (int m = 0; m < XO; ++m)
+        x[h][m] = !(p < b && m < w) ? (T)0
+                                    : (m < r1 ? (T)Uij[p + (int64_t)m * b] * sc[m] : Q2u[p + (int64_t)(m - r1) * b]);
+    }
+    for (int c = 0; c < rn; ++c) {
+      const int pc = perm[c];
+      const T* Uc = J + pc * w;
+      const T sv = sig[pc];
+#pragma unroll
+      for (int h = 0; h < NR; ++h) {
+        const int p = tid + NT * h;
+        T s = (T)0;
+#pragma unroll
+        for (int m = 0; m < XO; ++m)
+          if (m < w) s = fma(x[h][m], Uc[m], s);
+        if (p < b) Uij[p + (int64_t)c * b] = (double)(s * sv);
+      }
+    }
+    __syncthreads();
+  }
+  // wider cores: row blocks of RB rows staged through LDS (in place); register row block t of the panel
+  // covers rows [64 t, 64 t + 64), so t stays a compile-time index
   const int rbs = 64 * w <= L::FRONT ? 6 : (32 * w <= L::FRONT ? 5 : (16 * w <= L::FRONT ? 4 : 3));
   const int RB = 1 << rbs;
 #pragma unroll
   for (int t = 0; t < RPL; ++t) {
-    if (64 * t >= b) break;
+    if (rowout || 64 * t >= b) break;
     const int rend = min(64 * t + 64, b);
     for (int r0 = 64 * t; r0 < rend; r0 += RB) {
       // V_ij <- [V_ij, Q2v] V_core[:, perm]
@@ -925,13 +986,19 @@ hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t 
 template <int PM>
 using RecompT = typename std::conditional<PM == PM_FP64, double, float>::type;
 
+// dynamic LDS of a recompression launch
+template <int RPL, int NW, int CPW, int PM>
+constexpr size_t recomp_lds_bytes() {
+  return (size_t)RecompLayout<NW, CPW>::TOTAL * sizeof(RecompT<PM>);
+}
+
 template <int RPL, int NW, int CPW, int PM>
 static hipError_t recomp_attr() {
   static bool attr_set = false;
   hipError_t e = hipSuccess;
   if (!attr_set) {
     e = hipFuncSetAttribute((const void*)k_recompress<RPL, NW, CPW, PM>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(RecompLayout<NW, CPW>::TOTAL * sizeof(RecompT<PM>)));
+                            (int)recomp_lds_bytes<RPL, NW, CPW, PM>());
     attr_set = e == hipSuccess;
   }
   return e;
@@ -941,7 +1008,7 @@ template <int RPL, int NW, int CPW, int PM>
 static hipError_t launch_recomp_t(const SlotArgs& sa, int k, int cls, int panel, int grid, int src, hipStream_t s) {
   hipError_t e = recomp_attr<RPL, NW, CPW, PM>();
   if (e != hipSuccess) return e;
-  const size_t lds = (size_t)RecompLayout<NW, CPW>::TOTAL * sizeof(RecompT<PM>);
+  const size_t lds = recomp_lds_bytes<RPL, NW, CPW, PM>();
   hipLaunchKernelGGL((k_recompress<RPL, NW, CPW, PM>), dim3(grid), dim3(64 * NW), lds, s, sa, k, cls, panel, src);
   return hipGetLastError();
 }
@@ -951,7 +1018,7 @@ static hipError_t occ_t(int* out) {
   hipError_t e = recomp_attr<RPL, NW, CPW, PM>();
   if (e != hipSuccess) return e;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, (const void*)k_recompress<RPL, NW, CPW, PM>, 64 * NW,
-                                                      RecompLayout<NW, CPW>::TOTAL * sizeof(RecompT<PM>));
+                                                      recomp_lds_bytes<RPL, NW, CPW, PM>());
 }
 
 template <int PM>
