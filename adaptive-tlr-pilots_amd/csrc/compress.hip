@@ -244,7 +244,8 @@ __global__ __launch_bounds__(256) void k_rank_decide(double* __restrict__ Rm, in
                                                      double* __restrict__ U, double* __restrict__ V, int64_t fstride,
                                                      int rank_cap, int* __restrict__ rank, int* __restrict__ need,
                                                      int* __restrict__ retry_count, int* __restrict__ status,
-                                                     const int* __restrict__ forced, int mixed) {
+                                                     const int* __restrict__ forced, int mixed,
+                                                     double* __restrict__ tnorm) {
   const int64_t t = blockIdx.x;
   if (!need[t]) return;
   const double thr_rel = thr2[0], thr_abs = thr2[1];
@@ -336,7 +337,14 @@ __global__ __launch_bounds__(256) void k_rank_decide(double* __restrict__ Rm, in
     __syncthreads();
     if (3 * r * r <= rstride) round_pair_wg(Ut, b, b, Vt, b, b, r, thr_abs - tail[r], Rm + t * rstride, s_red);
   }
-  if (tid == 0) { rank[t] = r; need[t] = 0; }
+  if (tid == 0) {
+    // ||U||_F of the stored factors (V orthonormal): the kept singular values
+    double kept = 0.0;
+    for (int c = 0; c < r; ++c) kept += sig[perm[c]] * sig[perm[c]];
+    tnorm[t] = sqrt(kept);
+    rank[t] = r;
+    need[t] = 0;
+  }
 }
 
 __global__ void k_fill_int(int* p, int64_t count, int value) {
@@ -497,7 +505,7 @@ hipError_t launch_rank_decide(double* Rm, int64_t rstride, const double* Qrow, c
                               int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
                               const double* thr2, double* U, double* V, int64_t fstride, int rank_cap,
                               int* rank, int* need, int* retry_count, int* status, const int* forced, int mixed,
-                              int64_t ntile, hipStream_t s) {
+                              double* tnorm, int64_t ntile, hipStream_t s) {
   const size_t lds = rank_decide_lds(k);
   if (lds > 158 * 1024) return hipErrorInvalidValue;
   static bool attr_set = false;
@@ -507,7 +515,7 @@ hipError_t launch_rank_decide(double* Rm, int64_t rstride, const double* Qrow, c
   }
   hipLaunchKernelGGL(k_rank_decide, dim3((unsigned)ntile), dim3(256), lds, s, Rm, rstride, Qrow, Qcol, qstride, part,
                      b, k, kmax, pmin, rule_fixed, thr2, U, V, fstride, rank_cap, rank, need, retry_count,
-                     status, forced, mixed);
+                     status, forced, mixed, tnorm);
   return hipGetLastError();
 }
 

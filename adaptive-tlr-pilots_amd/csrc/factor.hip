@@ -345,6 +345,11 @@ __global__ __launch_bounds__(256) void k_trsm(SlotArgs sa, int k) {
     if (l == 0) { Wsh[a + c * r] = s; Wsh[c + a * r] = s; Wi[a + c * r] = s; Wi[c + a * r] = s; }
   }
   __syncthreads();
+  if (tid == 0) {   // tnorm: ||U_ik||_F (stored with V orthonormal) times ||V_ik||_F = sqrt(tr W_i) after the solve
+    double tr = 0.0;
+    for (int a = 0; a < r; ++a) tr += Wsh[a + a * r];
+    P.tnorm[tik] *= sqrt(tr);
+  }
   // X_i = U W, rows staged through LDS 32 at a time
   for (int r0 = 0; r0 < b; r0 += 32) {
     for (int e = tid; e < 32 * r; e += 256) {
@@ -524,6 +529,15 @@ struct RecompLayout {
 
 __device__ __forceinline__ int recomp_class(int r2) { return r2 <= 8 ? 0 : (r2 <= 16 ? 1 : (r2 <= 32 ? 2 : 3)); }
 
+// Append pair q of slot `slot` to list `level` of step k (panel: the pair updates the next panel column):
+// level 0 the FP64 redo list, 1 the FP32 redo list, 2 the reduced-first list.
+__device__ __forceinline__ void push_redo(const SlotArgs& sa, int k, int panel, int level, int q, int slot) {
+  int* cnt = sa.wlr_count + WL_STRIDE * ((k * 2 + panel) * REDO_LEVELS + level);
+  const int pos = atomicAdd(cnt, 1);
+  if (pos < sa.wlr_cap) sa.wlr[(int64_t)(panel * REDO_LEVELS + level) * sa.wlr_cap + pos] = make_int2(q, slot);
+  else atomicMax(&sa.slot(slot).status[ST_GUARD], 3);
+}
+
 // Work lists of step k: one thread per (pair, slot); pairs with r_ik, r_jk > 0 go to the list of their
 // class (by r_jk, and by the core width r_ij + r_jk for the LDS), the heavy ones (wide cores: r_ij + r_jk above the class's threshold) from the front and the rest
 // from the back, so that the persistent grid starts the long pairs first.  Ranks outside [0, rank_cap]
@@ -546,6 +560,26 @@ __global__ __launch_bounds__(256) void k_recomp_list(SlotArgs sa, int k, int npa
     return;
   }
   if (rik == 0 || r2 == 0) return;
+  if (sa.prec_mode != PM_FP64) {
+    // reduced precision: a pair goes to the reduced-first list only if it can pass the kernel's guard, judged
+    // with an upper bound of the core's scale (tnorm_ij + tnorm_ik tnorm_jk >= max(||core||_F,
+    // ||U_ik W||_F ||U_jk||_F)); the others run in FP64 on the class lists.  A replay follows its record.
+    const int rep = sa.replay[slot];
+    bool reduced;
+    if (rep >= 0) {
+      reduced = sa.rep_rpl[(int64_t)rep * sa.nlog + sa.rlog_off[k] + q] != PM_FP64;
+    } else if (sa.rule_fixed) {
+      // budget thr_rel ||core||_F against guard eps scale >= guard eps ||core||_F
+      reduced = sa.thr[2 * slot] >= sa.guard * EPS_FP32;
+    } else {
+      const double scale = P.tnorm[tri_index(i, j)] + P.tnorm[tri_index(i, k)] * P.tnorm[tri_index(j, k)];
+      reduced = sa.thr[2 * slot + 1] >= sa.guard * EPS_FP32 * (1.001 * scale);
+    }
+    if (reduced) {
+      push_redo(sa, k, j == k + 1 ? 1 : 0, 2, q, slot);
+      return;
+    }
+  }
   // the 4-wave classes keep cores up to 32 x 32 in LDS, the 8-wave ones up to 64 x 64: a wider core goes
   // to an 8-wave class, whose panel (32 columns) also holds its r_jk <= 16 new columns
   const int c = max(recomp_class(r2), r1 + r2 > 32 ? 2 : 0);
@@ -571,14 +605,6 @@ __global__ __launch_bounds__(256) void k_recomp_list(SlotArgs sa, int k, int npa
       list[heavy ? pos : cap - 1 - pos] = make_int2(q, slot);
     }
   }
-}
-
-// Append pair q of slot `slot` to the redo list (step k, panel, level) (level 0: FP64, 1: FP32).
-__device__ __forceinline__ void push_redo(const SlotArgs& sa, int k, int panel, int level, int q, int slot) {
-  int* cnt = sa.wlr_count + WL_STRIDE * ((k * 2 + panel) * 2 + level);
-  const int pos = atomicAdd(cnt, 1);
-  if (pos < sa.wlr_cap) sa.wlr[(int64_t)(panel * 2 + level) * sa.wlr_cap + pos] = make_int2(q, slot);
-  else atomicMax(&sa.slot(slot).status[ST_GUARD], 3);
 }
 
 // Block Gram-Schmidt of the panel against the r1 columns of A (global, or the LDS copy), applied twice:
@@ -761,6 +787,9 @@ __device__ __forceinline__ void recompress_pair(const SlotArgs& sa, int k, int q
     }
     if (rep >= 0) r = min(sa.rep_rlog[(int64_t)rep * sa.nlog + lo], w);   // frozen ranks
     if (r > sa.rank_cap) { atomicMax(&P.status[ST_RANKCAP], 1); r = sa.rank_cap; }
+    double kept = 0.0;   // ||U_ij||_F of the new factors (V_ij orthonormal)
+    for (int c = 0; c < r; ++c) kept += (double)sig[perm[c]] * (double)sig[perm[c]];
+    P.tnorm[tij] = sqrt(kept);
     P.rlog[lo] = r;
     P.rpl[lo] = PM;
     s_rank = r;
@@ -893,7 +922,8 @@ struct RecompWaves {
 // Entries are claimed one at a time from the list's claim counter, in list order (heavy pairs first), so a
 // work group that becomes resident late (its CU still held by another kernel) only takes what is left.
 // src 0: the class-cls list of step k (panel or the rest); src 1 / 2: the FP64 / FP32 redo list of step k
-// (pairs that a reduced-precision launch handed on; complete before this launch starts).
+// (pairs that a reduced-precision launch handed on; complete before this launch starts); src 3: the
+// reduced-first list of step k (complete once the list kernel is done).
 template <int RPL, int NW, int CPW, int PM>
 __global__ __launch_bounds__(64 * NW, (RecompWaves<NW, CPW>::value)) void k_recompress(SlotArgs sa, int k, int cls,
                                                                                         int panel, int src) {
@@ -908,12 +938,12 @@ __global__ __launch_bounds__(64 * NW, (RecompWaves<NW, CPW>::value)) void k_reco
     cnt = nh + __builtin_amdgcn_readfirstlane(cnts[1]);
     list = (panel ? sa.wlp : sa.wl) + (int64_t)cls * cap;
   } else {
-    const int lvl = src == 1 ? 0 : 1;
-    cnts = sa.wlr_count + WL_STRIDE * ((k * 2 + panel) * 2 + lvl);
+    const int lvl = src - 1;   // src 1: FP64 redo, 2: FP32 redo, 3: reduced-first
+    cnts = sa.wlr_count + WL_STRIDE * ((k * 2 + panel) * REDO_LEVELS + lvl);
     cap = sa.wlr_cap;
     nh = (int)min<int64_t>(__builtin_amdgcn_readfirstlane(cnts[0]), cap);
     cnt = nh;
-    list = sa.wlr + (int64_t)(panel * 2 + lvl) * cap;
+    list = sa.wlr + (int64_t)(panel * REDO_LEVELS + lvl) * cap;
   }
   double* scr = sa.scr_pool + (int64_t)blockIdx.x * sa.scr_stride;
   __shared__ int s_claim;
@@ -1070,8 +1100,8 @@ static hipError_t launch_class_pm(const SlotArgs& sa, int k, int cls, int panel,
 }
 
 // One persistent launch of class cls for step k (grid work groups, each looping over the class list, the
-// panel list when panel != 0, or a redo list (src)); scr is the launch's scratch pool, grid x scr_stride
-// doubles.  A redo list may hold pairs of any class: it runs on the widest class (cls 3).
+// panel list when panel != 0, or a redo / reduced-first list (src)); scr is the launch's scratch pool, grid x
+// scr_stride doubles.  Those lists may hold pairs of any class: they run on the widest class (cls 3).
 hipError_t launch_recomp_class(const SlotArgs& sa_in, int k, int cls, int panel, int grid, double* scr, hipStream_t s,
                                int pm, int src) {
   SlotArgs sa = sa_in;

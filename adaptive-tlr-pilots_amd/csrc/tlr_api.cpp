@@ -386,9 +386,10 @@ static int ctx_build(tlr_ctx* c, const double* S, const double* y, int64_t n, in
     c->wlp_cap = (int64_t)(c->nB > 0 ? c->nB : 1) * max_batch;
     ACHK(c->alloc(&c->wlp, (size_t)RCLASSES * c->wlp_cap));
     ACHK(c->alloc(&c->wlp_count, (size_t)WL_STRIDE * RCLASSES * c->nB));
-    // redo lists of the reduced-precision modes: (panel, level) lists of wl_cap entries, counters per step
-    ACHK(c->alloc(&c->wlr, (size_t)4 * c->wl_cap));
-    ACHK(c->alloc(&c->wlr_count, (size_t)WL_STRIDE * 4 * c->nB));
+    // redo and reduced-first lists of the reduced-precision modes: (panel, level) lists of wl_cap entries,
+    // counters per step
+    ACHK(c->alloc(&c->wlr, (size_t)2 * REDO_LEVELS * c->wl_cap));
+    ACHK(c->alloc(&c->wlr_count, (size_t)WL_STRIDE * 2 * REDO_LEVELS * c->nB));
   }
   c->slots.resize(max_batch);
   c->sstate.assign(max_batch, SlotState());
@@ -408,6 +409,7 @@ static int ctx_build(tlr_ctx* c, const double* S, const double* y, int64_t n, in
     ACHK(c->alloc(&p.status, (size_t)ST_COUNT));
     ACHK(c->alloc(&p.result, (size_t)4));
     ACHK(c->alloc(&p.prec, (size_t)nt));
+    ACHK(c->alloc(&p.tnorm, (size_t)nt));
   }
   // rank logs: offsets of each step's recompressions, one log per slot and max_batch replay records
   {
@@ -566,7 +568,7 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, int rule_fixed
     const double* Qcol = (last_op == 0) ? Qprev : M;
     HIPCHK(launch_rank_decide(bufR, rs, Qrow, Qcol, ms, part, b, k, kmax, PMIN, rule_fixed, thr2,
                               p.U, p.V, (int64_t)b * RANK_CAP, RANK_CAP, p.rank, need, c->d_retry + slot,
-                              p.status, forced, mixed, nt, s));
+                              p.status, forced, mixed, p.tnorm, nt, s));
   }
   HIPCHK(launch_copy_int(p.rank0, p.rank, nt, s));
   return TLR_OK;
@@ -607,9 +609,12 @@ static hipError_t launch_step_lookahead(tlr_ctx* c, const SlotArgs& sa, int k, i
   // classes run by side streams 1 and 2, in order; the third entry of stream 2 is a second launch over the
   // 17-32 lists (claims are atomic, so two launches share a list), which takes what stream 1 has not
   // started by the time stream 2 is done with its own classes
+  // In the reduced-precision modes the class lists hold the pairs routed to FP64 and run exactly as in plain
+  // FP64; stream 2 then runs the reduced-first list of each pass in the working precision on the widest
+  // class (after that class's own launch of the pass, whose scratch it reuses).
   static const int plan[2][3] = {{2, -1, -1}, {3, 1, 2}};
   const int pm = sa.prec_mode;
-  const int* grid = c->recomp_grid[pm];
+  const int* grid = c->recomp_grid[PM_FP64];
   for (int q = 1; q <= 2; ++q) {
     hipStream_t sq = c->cls_stream[q];
     if (ev && (e = hipStreamWaitEvent(sq, c->ev_list, 0)) != hipSuccess) return e;
@@ -618,21 +623,25 @@ static hipError_t launch_step_lookahead(tlr_ctx* c, const SlotArgs& sa, int k, i
         const int cls = plan[q - 1][u];
         if (cls < 0) continue;
         double* scr = u == 2 ? c->scr_help[pass] : (pass ? c->scr_pan[cls] : c->scr_cls[cls]);
-        if ((e = launch_recomp_class(sa, k, cls, pass, grid[cls], scr, sq, pm)) != hipSuccess) return e;
+        if ((e = launch_recomp_class(sa, k, cls, pass, grid[cls], scr, sq, PM_FP64)) != hipSuccess) return e;
       }
+      if (q == 2 && pm != PM_FP64 &&
+          (e = launch_recomp_class(sa, k, 3, pass, c->recomp_grid[pm][3], pass ? c->scr_pan[3] : c->scr_cls[3], sq,
+                                   pm, 3)) != hipSuccess)
+        return e;
       if (ev && pass == 1 && (e = hipEventRecord(c->ev_pan[q], sq)) != hipSuccess) return e;
     }
     if (ev && (e = hipEventRecord(c->ev_cls[q], sq)) != hipSuccess) return e;
   }
   if ((e = launch_potrf(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
   if ((e = launch_trtri(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
-  if ((e = launch_recomp_class(sa, k, 0, 1, grid[0], c->scr_pan[0], c->stream, pm)) != hipSuccess) return e;
-  if ((e = launch_recomp_class(sa, k, 0, 0, grid[0], c->scr_cls[0], c->stream, pm)) != hipSuccess) return e;
+  if ((e = launch_recomp_class(sa, k, 0, 1, grid[0], c->scr_pan[0], c->stream, PM_FP64)) != hipSuccess) return e;
+  if ((e = launch_recomp_class(sa, k, 0, 0, grid[0], c->scr_cls[0], c->stream, PM_FP64)) != hipSuccess) return e;
   for (int q = 1; q <= 2 && ev; ++q)
     if ((e = hipStreamWaitEvent(c->stream, c->ev_pan[q], 0)) != hipSuccess) return e;
   // reduced precision: the pairs the guard handed on, once every launch that could add to their lists is
   // done (BF16 -> FP32 -> FP64); the widest class takes any pair, on the scratch of the finished panel (or
-  // rest) launches of that class
+  // rest) launches of that class (the reduced-first launch of stream 2 among them)
   if (pm != PM_FP64 && (e = launch_redo(c, sa, k, 1)) != hipSuccess) return e;
   if ((e = launch_trsm(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
   if ((e = launch_diag_update(sa, k + 1, nslot, c->stream)) != hipSuccess) return e;
@@ -768,7 +777,7 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
   HIPCHK(launch_fill_int(c->d_retry, c->max_batch, 0, st));
   HIPCHK(launch_fill_int(c->wl_count, (int64_t)WL_STRIDE * RCLASSES * c->nB, 0, st));
   HIPCHK(launch_fill_int(c->wlp_count, (int64_t)WL_STRIDE * RCLASSES * c->nB, 0, st));
-  HIPCHK(launch_fill_int(c->wlr_count, (int64_t)WL_STRIDE * 4 * c->nB, 0, st));
+  HIPCHK(launch_fill_int(c->wlr_count, (int64_t)WL_STRIDE * 2 * REDO_LEVELS * c->nB, 0, st));
   HIPCHK(hipMemcpyAsync(c->d_thr, c->h_thr, sizeof(double) * 2 * m, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(c->d_mp, c->h_mp, sizeof(MaternParams) * m, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(c->d_replay, c->h_replay, sizeof(int) * m, hipMemcpyHostToDevice, st));

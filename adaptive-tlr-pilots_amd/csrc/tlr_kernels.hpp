@@ -14,6 +14,8 @@ enum { ST_RANKCAP = 0, ST_FAILED = 1, ST_GUARD = 2, ST_COUNT = 4 };
 constexpr int RCLASSES = 4;
 // counters per (step, class) of a work list: heavy entries, light entries, entries claimed, unused
 constexpr int WL_STRIDE = 4;
+// per-step lists beside the class lists: FP64 redo, FP32 redo, reduced-first
+constexpr int REDO_LEVELS = 3;
 
 // Device buffers of one slot (one theta being evaluated).  Layouts (all FP64 column-major):
 //   D      nB tiles of b x b: diagonal tiles, factored in place into L_kk (lower)
@@ -40,6 +42,9 @@ struct SlotPtrs {
   int* rlog;          // rank chosen by every recompression, at rlog_off[k] + pair index
   int* rpl;           // arithmetic of every recompression (PM_FP64, PM_FP32, PM_BF16; -1 not run), as rlog
   int* prec;          // storage precision (8, 16, 32, 64) of every factor tile, mixed rule
+  double* tnorm;      // per off-diagonal tile: a bound of ||U||_F ||V||_2 of its current factors (the initial
+                      // compression and every recompression store ||U||_F with V orthonormal; TRSM scales it
+                      // by ||V||_F), from which the list routes reduced-precision pairs
 };
 
 // Arithmetic of the factorization's recompressions (the products W = V_ik^T V_jk and U_ik W, the projections,
@@ -50,6 +55,9 @@ enum { PM_FP64 = 0, PM_FP32 = 1, PM_BF16 = 2 };
 // A reduced-precision recompression goes ahead only when its budget is at least `guard` unit roundoffs of
 // its working precision times the scale of the core (the larger of ||core||_F and the bound ||U_ik W||_F
 // ||U_jk||_F of the update it absorbs); otherwise the pair goes to a redo list of a higher precision.
+// The work lists apply the same test beforehand with an upper bound of that scale from the tiles' norms
+// (tnorm_ij + tnorm_ik tnorm_jk): pairs that cannot pass go straight to the FP64 class lists, the others to
+// the reduced-first list, whose kernel still checks the exact scale.
 constexpr double GUARD_DEFAULT = 16384.0;
 constexpr double EPS_FP32 = 1.1920928955078125e-07, EPS_BF16 = 3.90625e-03;   // 2^-23, 2^-8
 
@@ -65,8 +73,9 @@ struct SlotArgs {
   int2* wlp;            // the same for the pairs (i, k+1) that update the next panel column (run first)
   int* wlp_count;
   int64_t wlp_cap;
-  int2* wlr;            // redo lists: (step, panel, level) -> entries; level 0 redo in FP64, 1 in FP32
-  int* wlr_count;       // nB x 2 (panel) x 2 (level) x WL_STRIDE counters
+  int2* wlr;            // (panel, level) lists: level 0 redo in FP64, 1 redo in FP32, 2 the pairs routed to
+                        // the reduced precision at list time (REDO_LEVELS lists of wlr_cap per panel flag)
+  int* wlr_count;       // nB x 2 (panel) x REDO_LEVELS x WL_STRIDE counters
   int64_t wlr_cap;
   int prec_mode;        // PM_FP64, PM_FP32 or PM_BF16 (arithmetic of the class launches)
   double guard;         // unit roundoffs of headroom a reduced-precision pair needs (GUARD_DEFAULT)
@@ -102,7 +111,7 @@ hipError_t launch_rank_decide(double* Rm, int64_t rstride, const double* Qrow, c
                               int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
                               const double* thr2, double* U, double* V, int64_t fstride, int rank_cap,
                               int* rank, int* need, int* retry_count, int* status, const int* forced, int mixed,
-                              int64_t ntile, hipStream_t s);
+                              double* tnorm, int64_t ntile, hipStream_t s);
 hipError_t launch_warm_sketch(const double* stash, const int64_t* off, const int* r0, const double* omega, double* X,
                               int64_t xstride, int b, int k, const int* active, int64_t ntile, hipStream_t s);
 hipError_t launch_fill_int(int* p, int64_t count, int value, hipStream_t s);
@@ -116,7 +125,8 @@ hipError_t launch_trtri(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_trsm(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_diag_update(const SlotArgs& sa, int k, int nslot, hipStream_t s);
 hipError_t launch_recomp_list(const SlotArgs& sa, int k, int nslot, hipStream_t s);
-// pm: arithmetic (PM_*); src: 0 the class's work list, 1 the FP64 redo list, 2 the FP32 redo list
+// pm: arithmetic (PM_*); src: 0 the class's work list, 1 the FP64 redo list, 2 the FP32 redo list, 3 the
+// reduced-first list
 hipError_t launch_recomp_class(const SlotArgs& sa, int k, int cls, int panel, int grid, double* scr, hipStream_t s,
                                int pm = PM_FP64, int src = 0);
 // resident work groups per CU of each recompression class in arithmetic pm (b selects the row-panel variant)
