@@ -55,6 +55,19 @@ __global__ void k_gen_diag(const double* __restrict__ sx, const double* __restri
 // generated entry for 64 columns, and the entry generation (VALU) overlaps the matrix pipe.  On 3160-tile
 // tests its outputs equal those of a sequential FMA loop over the inner index bit for bit.
 constexpr int MM_ROWS = 128, MM_COLS = 64, MM_QC = 64, MM_LD = MM_COLS + 1;
+
+// Work group -> (tile t, block blk) for 1-D grids of `per` blocks per tile.  The dispatcher deals work groups
+// to the eight XCDs in turn (id mod 8), so the blocks of a tile are placed 8 apart in id order: every block of
+// tile t runs on XCD t mod 8 and the operands the blocks share are fetched from HBM once, into that XCD's L2.
+// Grid: ceil(ntile / 8) * 8 * per work groups; returns false past the last tile.
+__device__ __forceinline__ bool xcd_tile_block(int64_t ntile, int per, int64_t* t, int* blk) {
+  const int64_t id = blockIdx.x;
+  const int64_t g = id / (8 * (int64_t)per);
+  const int loc = (int)(id - g * 8 * per);
+  *t = g * 8 + (loc & 7);
+  *blk = loc >> 3;
+  return *t < ntile;
+}
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256, 4) void k_matern_product(const double* __restrict__ sx, const double* __restrict__ sy,
@@ -62,18 +75,22 @@ __global__ __launch_bounds__(256, 4) void k_matern_product(const double* __restr
                                                            const MaternParams* __restrict__ mpp, int op,
                                                            const double* __restrict__ X, int64_t xstride,
                                                            double* __restrict__ out, int64_t ostride, int k,
-                                                           const int* __restrict__ active, int64_t t0) {
-  const int64_t t = t0 + blockIdx.z;
+                                                           const int* __restrict__ active, int64_t ntile) {
+  const int gx = (b + MM_ROWS - 1) / MM_ROWS, gy = (k + MM_COLS - 1) / MM_COLS;
+  int64_t t;
+  int blk;
+  if (!xcd_tile_block(ntile, gx * gy, &t, &blk)) return;
   if (active && !active[t]) return;
+  const int rblk = blk % gx, cblk = blk / gx;
   const MaternParams mp = *mpp;
   int i, j;
   tri_decode(t, &i, &j);
   if (op) { const int x = i; i = j; j = x; }
   const int64_t r0g = (int64_t)i * b, q0g = (int64_t)j * b;
   const int mr = (int)min<int64_t>(b, n - r0g), mq = (int)min<int64_t>(b, n - q0g);
-  const int cb = blockIdx.y * MM_COLS;
+  const int cb = cblk * MM_COLS;
   const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63, lr = ln & 15, lq = ln >> 4;
-  const int p0 = blockIdx.x * MM_ROWS + 32 * wv;   // this wave's first row
+  const int p0 = rblk * MM_ROWS + 32 * wv;   // this wave's first row
   __shared__ double Xs[MM_QC * MM_LD];             // X chunk, [inner][column]
   __shared__ double qxs[MM_QC], qys[MM_QC];
   // the lane's two rows (sites); rows past the tile generate zeros
@@ -164,16 +181,18 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ sx,
                                                   int64_t n, int b, const MaternParams* __restrict__ mpp,
                                                   const double* __restrict__ A, const double* __restrict__ B,
                                                   int64_t stride, int k, double* __restrict__ part,
-                                                  const int* __restrict__ active, int64_t t0) {
-  const int64_t t = t0 + blockIdx.z;
+                                                  const int* __restrict__ active, int64_t ntile) {
+  const int nbb = (b + 63) / 64;
+  int64_t t;
+  int blk;
+  if (!xcd_tile_block(ntile, nbb * nbb, &t, &blk)) return;
   if (active && !active[t]) return;
   const MaternParams mp = *mpp;
   int i, j;
   tri_decode(t, &i, &j);
   const int64_t r0g = (int64_t)i * b, q0g = (int64_t)j * b;
   const int mr = (int)min<int64_t>(b, n - r0g), mq = (int)min<int64_t>(b, n - q0g);
-  const int nbb = (b + 63) / 64;
-  const int pb = blockIdx.x * 64, qb = blockIdx.y * 64;
+  const int pb = (blk % nbb) * 64, qb = (blk / nbb) * 64;
   __shared__ double As[32][64];
   __shared__ double Bs[32][64];
   __shared__ double red[2][4];
@@ -229,7 +248,6 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ sx,
   if (lane_id() == 0) { red[0][wave_id()] = res2; red[1][wave_id()] = fro2; }
   __syncthreads();
   if (tid == 0) {
-    const int64_t blk = (int64_t)blockIdx.y * nbb + blockIdx.x;
     part[(t * nbb * nbb + blk) * 2 + 0] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
     part[(t * nbb * nbb + blk) * 2 + 1] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
   }
@@ -449,12 +467,11 @@ hipError_t launch_matern_product(const double* sx, const double* sy, int64_t n, 
                                  const MaternParams* mp,
                                  int op, const double* X, int64_t xstride, double* out, int64_t ostride, int k,
                                  const int* active, hipStream_t s) {
-  for (int64_t t0 = 0; t0 < ntile; t0 += TILE_CHUNK) {
-    const int64_t nt = ntile - t0 < TILE_CHUNK ? ntile - t0 : TILE_CHUNK;
-    dim3 grid(nblk(b, MM_ROWS), nblk(k, MM_COLS), (unsigned)nt);
-    hipLaunchKernelGGL(k_matern_product, grid, dim3(256), 0, s, sx, sy, n, b, mp, op, X, xstride, out, ostride, k,
-                       active, t0);
-  }
+  if (ntile <= 0) return hipSuccess;
+  const int64_t groups = (ntile + 7) / 8;
+  const int per = nblk(b, MM_ROWS) * nblk(k, MM_COLS);
+  hipLaunchKernelGGL(k_matern_product, dim3((unsigned)(groups * 8 * per)), dim3(256), 0, s, sx, sy, n, b, mp, op, X,
+                     xstride, out, ostride, k, active, ntile);
   return hipGetLastError();
 }
 
@@ -489,11 +506,10 @@ hipError_t launch_residual(const double* sx, const double* sy, int64_t n, int b,
                            const double* A, const double* B, int64_t stride, int k, double* part, const int* active,
                            hipStream_t s) {
   const int nbb = nblk(b, 64);
-  for (int64_t t0 = 0; t0 < ntile; t0 += TILE_CHUNK) {
-    const int64_t nt = ntile - t0 < TILE_CHUNK ? ntile - t0 : TILE_CHUNK;
-    dim3 grid(nbb, nbb, (unsigned)nt);
-    hipLaunchKernelGGL(k_residual, grid, dim3(256), 0, s, sx, sy, n, b, mp, A, B, stride, k, part, active, t0);
-  }
+  if (ntile <= 0) return hipSuccess;
+  const int64_t groups = (ntile + 7) / 8;
+  hipLaunchKernelGGL(k_residual, dim3((unsigned)(groups * 8 * nbb * nbb)), dim3(256), 0, s, sx, sy, n, b, mp, A, B,
+                     stride, k, part, active, ntile);
   return hipGetLastError();
 }
 
