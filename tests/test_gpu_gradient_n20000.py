@@ -52,13 +52,21 @@ def test_gradient_six_fields(tl, fx):
         # small difference of large terms, and the rule's approximation moves it by a share of those terms
         rel = np.abs(g - g_d) / max(1.0, float(np.abs(g_d).max()))
         worst_dense = max(worst_dense, float(rel.max()))
-        # reduced-precision arithmetic (frozen ranks and arithmetic, step 1e-3) against FP64
+        # reduced-precision arithmetic (frozen ranks and arithmetic) against FP64 with the same difference step
+        # (1e-3: the arithmetic's error, not the step's truncation error, is what is compared), and against the
+        # FP64 gradient at the default step (both errors together).  An FP32 error delta in log L moves a
+        # difference quotient by up to delta / h in every component (delta ~ 1e-5 here), so the distances are
+        # taken relative to the gradient's largest component, as for the dense gradient above.
         r32 = ctx.loglik_grad_batch(th[None, :], "adaptive", 0.9, h=1e-3, precision="fp32", ranks=False)
-        d32 = np.abs(r32["grad"][0] - g) / np.maximum(1.0, np.abs(g))
-        print(f"  gpu {g} oracle fd {g_fd} dense {g_d}: vs dense {rel}; fp32 vs fp64 {d32}; "
-              f"dlogL fp32 {r32['loglik'][0] - ll:.2e}", flush=True)
+        r64h = ctx.loglik_grad_batch(th[None, :], "adaptive", 0.9, h=1e-3, precision="fp64", ranks=False)
+        scale = max(1.0, float(np.abs(g).max()))
+        d32 = np.abs(r32["grad"][0] - r64h["grad"][0]) / scale
+        dstep = np.abs(r32["grad"][0] - g) / scale
+        print(f"  gpu {g} oracle fd {g_fd} dense {g_d}: vs dense {rel}; fp32 vs fp64 (h=1e-3) {d32}; "
+              f"fp32 (h=1e-3) vs fp64 (h=1e-5) {dstep}; dlogL fp32 {r32['loglik'][0] - ll:.2e}", flush=True)
         assert abs(r32["loglik"][0] - ll) <= 1e-2
         assert np.all(d32 <= 1e-3)
+        assert np.all(dstep <= 1e-2)
         ctx.close()
     # the adaptive rule's own approximation separates the TLR gradient from the dense one
     print("largest relative distance to the dense gradient:", worst_dense, flush=True)
