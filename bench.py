@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # counter calibrated against a device copy of known size, divided by steps and theta); measured with
 # rocprofv3 --pmc on this workload in plain FP64
 # (profiles/factorization_and_compression_n40000_b500_pmc_traffic.md).
-STEP_TRAFFIC_PER_THETA = 0.395e9
+STEP_TRAFFIC_PER_THETA = 0.435e9
 # central-difference step of the gradient in log theta: 1e-5 in FP64; the reduced-precision modes take a
 # larger step so that their rounding (not smooth in theta) stays small against the difference
 GRAD_H = {"fp64": 1e-5, "fp32": 1e-3, "bf16": 1e-3}
