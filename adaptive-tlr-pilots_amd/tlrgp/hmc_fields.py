@@ -47,7 +47,8 @@ def main(argv=None):
         print(f"# field {path}: {names[-1]}", flush=True)
     f = inference.concurrent_loglik_grad(parts)
     t0 = time.perf_counter()
-    res = inference.hmc(f, np.concatenate(starts), a.iters, step=a.step, n_leapfrog=a.leapfrog, seed=a.seed)
+    res = inference.hmc(f, np.concatenate(starts), a.iters, step=a.step, n_leapfrog=a.leapfrog, seed=a.seed,
+                        log=lambda m: print(f"# {m} ({time.perf_counter() - t0:.0f} s)", flush=True))
     wall = time.perf_counter() - t0
     out = dict(fields=names, precision=a.precision, iters=a.iters, leapfrog=a.leapfrog, step=a.step,
                evaluations=res["evaluations"], evals_per_s=res["evaluations"] / res["seconds"],

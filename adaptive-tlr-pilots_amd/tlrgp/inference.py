@@ -380,13 +380,14 @@ def run_chains_batched(lik_batch, starts, seeds, iters, burn, ckpt=None, ckpt_ev
             for c in range(M)]
 
 
-def hmc(loglik_grad, starts, iters, step=0.01, n_leapfrog=10, seed=0, ckpt=None, ckpt_every=50):
+def hmc(loglik_grad, starts, iters, step=0.01, n_leapfrog=10, seed=0, ckpt=None, ckpt_every=50, log=None):
     """Batched HMC over M chains in log theta with the box prior LO..HI.
 
     loglik_grad(log_thetas (M, 3)) -> (ll (M,), grad (M, 3)) evaluates every chain at once (ll = -inf when
     the factorization fails; such proposals are rejected).  Each iteration runs n_leapfrog leapfrog steps of
     size `step` (one batched evaluation per step).  Returns dict with the chains (iters, M, 3), log L,
-    acceptance per chain, evaluation count and per-leapfrog-step latencies."""
+    acceptance per chain, evaluation count and per-leapfrog-step latencies.  log(message), when given, is
+    called after every iteration."""
     rng = np.random.default_rng(seed)
     q = np.array(starts, float)
     M = q.shape[0]
@@ -435,6 +436,9 @@ def hmc(loglik_grad, starts, iters, step=0.01, n_leapfrog=10, seed=0, ckpt=None,
         if ckpt and (it + 1) % ckpt_every == 0:
             _save_ckpt(ckpt, it=it + 1, chain=chain, lls=lls, q=q, acc=acc,
                        rng_state=np.array(json.dumps(rng.bit_generator.state)))
+        if log:
+            log(f"iteration {it + 1}/{iters}: {len(lat)} batched evaluations, last leapfrog step {lat[-1]:.3f} s, "
+                f"accepted {int(ok.sum())}/{M}")
     lat = np.array(lat)
     return dict(chain=chain, lls=lls, acceptance=(acc / max(iters - it0, 1)).tolist(), evaluations=len(lat) * M,
                 step_latency_p50=float(np.percentile(lat, 50)) if lat.size else None,
