@@ -253,6 +253,92 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ sx,
   }
 }
 
+// ------------------------------------------------------------------- dense analysis of the rule's tiles
+// Sigma(theta) of all N sites, N x N column-major (leading dimension ldk): sigma2 matern32 + sigma2 eta on the
+// diagonal, entry for entry the values the diagonal tiles and the Matern products use (codes/11:61-65).
+__global__ __launch_bounds__(256) void k_dense_cov(const double* __restrict__ sx, const double* __restrict__ sy,
+                                                   int64_t n, MaternParams mp, double* __restrict__ K, int64_t ldk) {
+  const int64_t q = blockIdx.x;                           // column
+  for (int64_t p = (int64_t)blockIdx.y * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.y * 256) {
+    double v = matern32(mp, sx[p], sy[p], sx[q], sy[q]);
+    if (p == q) v += mp.nugget;
+    K[p + q * ldk] = v;
+  }
+}
+
+// The assembled approximation's difference Delta = Sigma^TLR - Sigma on the off-diagonal tiles (codes/11:74-109:
+// every tile (i, j), i > j, replaced by its rank-r truncation U V^T): Delta_ij = U_ij V_ij^T - T_ij written to D
+// at (i, j) and transposed at (j, i) (column-major, leading dimension ldd; diagonal tiles untouched), and
+// ||Delta_ij||_F^2 to err2[t].  One work group per tile, entries in a fixed order (reproducible sums).
+__global__ __launch_bounds__(256) void k_assemble_delta(const double* __restrict__ sx, const double* __restrict__ sy,
+                                                        int64_t n, int b, const MaternParams* __restrict__ mpp,
+                                                        const double* __restrict__ U, const double* __restrict__ V,
+                                                        int64_t fstride, const int* __restrict__ rank,
+                                                        double* __restrict__ D, int64_t ldd,
+                                                        double* __restrict__ err2) {
+  const int64_t t = blockIdx.x;
+  const MaternParams mp = *mpp;
+  int i, j;
+  tri_decode(t, &i, &j);
+  const int64_t r0 = (int64_t)i * b, q0 = (int64_t)j * b;
+  const int mr = (int)min<int64_t>(b, n - r0), mq = (int)min<int64_t>(b, n - q0);
+  const int r = rank[t];
+  const double* Ut = U + t * fstride;
+  const double* Vt = V + t * fstride;
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x; e < (int64_t)mr * mq; e += 256) {
+    const int p = (int)(e % mr), q = (int)(e / mr);
+    double s = 0.0;
+    for (int c = 0; c < r; ++c) s = fma(Ut[p + (int64_t)c * b], Vt[q + (int64_t)c * b], s);
+    const double d = s - matern32(mp, sx[r0 + p], sy[r0 + p], sx[q0 + q], sy[q0 + q]);
+    D[(r0 + p) + (q0 + q) * ldd] = d;
+    D[(q0 + q) + (r0 + p) * ldd] = d;
+    acc = fma(d, d, acc);
+  }
+  acc = wave_sum(acc);
+  if (lane_id() == 0) red[wave_id()] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) err2[t] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// ||A_ij||_F of every lower tile (i > j) of a dense N x N column-major matrix (leading dimension lda), tiles of
+// b rows as tile_bounds(N, b), in tile order; one work group per tile, fixed-order sums.
+__global__ __launch_bounds__(256) void k_tile_fro(const double* __restrict__ A, int64_t lda, int64_t n, int b,
+                                                  double* __restrict__ out) {
+  const int64_t t = blockIdx.x;
+  int i, j;
+  tri_decode(t, &i, &j);
+  const int64_t r0 = (int64_t)i * b, q0 = (int64_t)j * b;
+  const int mr = (int)min<int64_t>(b, n - r0), mq = (int)min<int64_t>(b, n - q0);
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x; e < (int64_t)mr * mq; e += 256) {
+    const double v = A[(r0 + e % mr) + (q0 + e / mr) * lda];
+    acc = fma(v, v, acc);
+  }
+  acc = wave_sum(acc);
+  if (lane_id() == 0) red[wave_id()] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[t] = sqrt(((red[0] + red[1]) + red[2]) + red[3]);
+}
+
+// Symmetric part of a square N x N matrix in place: mode 0 copies the lower triangle over the upper one, mode 1
+// replaces both by their mean (A + A^T) / 2.  Entry (p, q), p > q, one thread each.
+__global__ __launch_bounds__(256) void k_symmetrize(double* __restrict__ A, int64_t lda, int64_t n, int mode) {
+  const int64_t q = blockIdx.y;
+  for (int64_t p = q + 1 + (int64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256) {
+    const double lo = A[p + q * lda];
+    if (mode == 0) {
+      A[q + p * lda] = lo;
+    } else {
+      const double m = 0.5 * (lo + A[q + p * lda]);
+      A[p + q * lda] = m;
+      A[q + p * lda] = m;
+    }
+  }
+}
+
 // ------------------------------------------------------------------------------ rank decision
 __global__ __launch_bounds__(256) void k_rank_decide(double* __restrict__ Rm, int64_t rstride,
                                                      const double* __restrict__ Qrow, const double* __restrict__ Qcol,
@@ -532,6 +618,36 @@ hipError_t launch_rank_decide(double* Rm, int64_t rstride, const double* Qrow, c
   hipLaunchKernelGGL(k_rank_decide, dim3((unsigned)ntile), dim3(256), lds, s, Rm, rstride, Qrow, Qcol, qstride, part,
                      b, k, kmax, pmin, rule_fixed, thr2, U, V, fstride, rank_cap, rank, need, retry_count,
                      status, forced, mixed, tnorm);
+  return hipGetLastError();
+}
+
+hipError_t launch_dense_cov(const double* sx, const double* sy, int64_t n, MaternParams mp, double* K, int64_t ldk,
+                            hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int rows = nblk(n, 256) < 1024 ? nblk(n, 256) : 1024;
+  hipLaunchKernelGGL(k_dense_cov, dim3((unsigned)n, (unsigned)rows), dim3(256), 0, s, sx, sy, n, mp, K, ldk);
+  return hipGetLastError();
+}
+
+hipError_t launch_assemble_delta(const double* sx, const double* sy, int64_t n, int b, const MaternParams* mp,
+                                 const double* U, const double* V, int64_t fstride, const int* rank, double* D,
+                                 int64_t ldd, double* err2, int64_t ntile, hipStream_t s) {
+  if (ntile <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_assemble_delta, dim3((unsigned)ntile), dim3(256), 0, s, sx, sy, n, b, mp, U, V, fstride, rank,
+                     D, ldd, err2);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_fro(const double* A, int64_t lda, int64_t n, int b, double* out, int64_t ntile, hipStream_t s) {
+  if (ntile <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tile_fro, dim3((unsigned)ntile), dim3(256), 0, s, A, lda, n, b, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_symmetrize(double* A, int64_t lda, int64_t n, int mode, hipStream_t s) {
+  if (n <= 1 || n > 65535) return n <= 1 ? hipSuccess : hipErrorInvalidValue;
+  const int rows = nblk(n, 256) < 256 ? nblk(n, 256) : 256;
+  hipLaunchKernelGGL(k_symmetrize, dim3((unsigned)rows, (unsigned)n), dim3(256), 0, s, A, lda, n, mode);
   return hipGetLastError();
 }
 

@@ -7,6 +7,8 @@
 //             diagonal updates(k) -> recompressions(k) (four width classes on three streams, with the next
 //             step's POTRF, inverse, TRSM and diagonal update overlapped)
 //   final reductions; one synchronisation and a small copy back.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -1270,6 +1272,248 @@ int tlr_loglik(tlr_ctx* ctx, const double theta[3], int32_t rule, double value, 
   const int rc = tlr_loglik_batch(ctx, 1, theta, rule, value, precision, out3, &st, &fa, ranks_sigma, ranks_factor);
   if (failed_at) *failed_at = fa;
   return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ dense analysis of the rule's tiles
+// The error terms of the assembled approximation Sigma^TLR (codes/36_likelihood_error_terms.py:75-136) need
+// the dense N x N matrices Sigma and Delta = Sigma^TLR - Sigma.  Sigma, Delta (from the rule's compressed
+// tiles), tile norms and symmetric parts come from this library's kernels; Cholesky factorizations, the
+// inverse, triangular solves, products and the symmetric eigensolver are rocSOLVER / rocBLAS calls on the
+// context's stream.
+
+namespace {
+
+struct DevArr {   // device allocation released on every return
+  double* p = nullptr;
+  ~DevArr() { if (p) (void)hipFree(p); }
+};
+
+struct Blas {
+  rocblas_handle h = nullptr;
+  ~Blas() { if (h) (void)rocblas_destroy_handle(h); }
+};
+
+// the rule's tiles at theta into D (Delta on the off-diagonal tiles; D zeroed first), ||Delta_ij||_F per tile
+// and the ranks (tri_index order): the initial compression of an evaluation at theta in slot 0
+int assembled_delta(tlr_ctx* c, const double theta[3], int32_t rule, double value, double* D,
+                    std::vector<double>& err, std::vector<int>& rk) {
+  const int64_t nt = c->ntile;
+  hipStream_t st = c->stream;
+  c->sstate[0].valid = false;   // slot 0's resident factor is given up
+  const double bmax = (double)(c->n < c->b ? c->n : c->b);
+  c->h_thr[0] = rule == TLR_RULE_FIXED ? value : 0.0;
+  c->h_thr[1] = rule == TLR_RULE_FIXED ? 0.0 : value * bmax * theta[0] * theta[2] / (double)c->n;
+  c->h_mp[0].sigma2 = theta[0];
+  c->h_mp[0].inv_range_sqrt3 = sqrt(3.0) / theta[1];
+  c->h_mp[0].nugget = theta[0] * theta[2];
+  const SlotPtrs& p = c->slots[0].p;
+  HIPCHK(hipMemcpyAsync(c->d_thr, c->h_thr, sizeof(double) * 2, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->d_mp, c->h_mp, sizeof(MaternParams), hipMemcpyHostToDevice, st));
+  HIPCHK(launch_fill_int(p.status, ST_COUNT, 0, st));
+  HIPCHK(launch_fill_int(p.status + ST_FAILED, 1, 0x7fffffff, st));
+  HIPCHK(launch_fill_int(c->d_retry, 1, 0, st));
+  HIPCHK(hipMemsetAsync(D, 0, sizeof(double) * c->n * c->n, st));
+  int rc = compress_slot(c, 0, p, rule == TLR_RULE_FIXED, nullptr, 0, -1, 0);
+  if (rc < 0) {
+    (void)hipStreamSynchronize(st);
+    return rc;
+  }
+  double* err2 = c->bufR[0];   // the compression's cores are dead by now: nt doubles of tile errors
+  HIPCHK(launch_assemble_delta(c->sx, c->sy, c->n, c->b, c->d_mp, p.U, p.V, (int64_t)c->b * RANK_CAP, p.rank, D,
+                               c->n, err2, nt, st));
+  err.assign(nt > 0 ? nt : 1, 0.0);
+  rk.assign(nt > 0 ? nt : 1, 0);
+  int sts[ST_COUNT];
+  if (nt > 0) {
+    HIPCHK(hipMemcpyAsync(err.data(), err2, nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(rk.data(), p.rank, nt * sizeof(int), hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipMemcpyAsync(sts, p.status, sizeof(sts), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (sts[ST_GUARD]) return set_err(TLR_ERR_HIP, "a compression guard tripped (non-finite tile)");
+  if (sts[ST_RANKCAP]) return set_err(TLR_ERR_RANKCAP, "a tile needs more columns than the rank capacity");
+  for (int64_t t = 0; t < nt; ++t) err[t] = sqrt(err[t]);
+  return TLR_OK;
+}
+
+#define RBCHK(x)                                                                   \
+  do {                                                                             \
+    rocblas_status rs_ = (x);                                                      \
+    if (rs_ != rocblas_status_success)                                             \
+      return set_err(TLR_ERR_HIP, std::string("rocBLAS/rocSOLVER: ") + rocblas_status_to_string(rs_)); \
+  } while (0)
+
+// sum over an array of m doubles (m may exceed the 32-bit counts of the BLAS calls) of x*y or x*x
+int long_dot(rocblas_handle h, int64_t m, const double* x, const double* y, double* out) {
+  const int64_t chunk = (int64_t)1 << 30;
+  double acc = 0.0;
+  for (int64_t o = 0; o < m; o += chunk) {
+    const rocblas_int len = (rocblas_int)(m - o < chunk ? m - o : chunk);
+    double r = 0.0;
+    RBCHK(rocblas_ddot(h, len, x + o, 1, y + o, 1, &r));
+    acc += r;
+  }
+  *out = acc;
+  return TLR_OK;
+}
+
+// log L = -(y^T alpha + 2 sum log diag L + N log 2 pi) / 2 from a Cholesky factor L (lower, in A) of Sigma;
+// alpha in the n-vector a
+int dense_loglik(tlr_ctx* c, rocblas_handle h, const double* A, double* a, double* ll) {
+  const int64_t n = c->n;
+  hipStream_t st = c->stream;
+  HIPCHK(hipMemcpyAsync(a, c->y, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  RBCHK(rocsolver_dpotrs(h, rocblas_fill_lower, (rocblas_int)n, 1, const_cast<double*>(A), (rocblas_int)n, a,
+                         (rocblas_int)n));
+  std::vector<double> dg(n);
+  HIPCHK(hipMemcpy2DAsync(dg.data(), sizeof(double), A, (n + 1) * sizeof(double), sizeof(double), n,
+                          hipMemcpyDeviceToHost, st));
+  double ya = 0.0;
+  RBCHK(rocblas_ddot(h, (rocblas_int)n, c->y, 1, a, 1, &ya));
+  HIPCHK(hipStreamSynchronize(st));
+  double ld = 0.0;
+  for (int64_t i = 0; i < n; ++i) ld += log(dg[i]);
+  *ll = -0.5 * (ya + 2.0 * ld + (double)n * 1.8378770664093453);
+  return TLR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tlr_bound_terms(tlr_ctx* c, const double theta[3], int32_t nrules, const int32_t* rules, const double* values,
+                    double dense[2], double* kinv_tiles, double* cov_tiles, double* alpha_blocks, double* terms,
+                    double* tile_err, int32_t* ranks) {
+  if (!c || !theta || nrules < 0 || (nrules > 0 && (!rules || !values || !terms)) || !dense)
+    return set_err(TLR_ERR_ARG, "invalid argument");
+  if (!(theta[0] > 0.0) || !(theta[1] > 0.0) || !(theta[2] >= 0.0)) return set_err(TLR_ERR_ARG, "theta must be positive");
+  for (int q = 0; q < nrules; ++q) {
+    if (rules[q] != TLR_RULE_FIXED && rules[q] != TLR_RULE_ADAPTIVE)
+      return set_err(TLR_ERR_ARG, "rule must be fixed or adaptive");
+    if (!(values[q] > 0.0)) return set_err(TLR_ERR_ARG, "rule value must be positive");
+  }
+  if (c->arena) return set_err(TLR_ERR_ARG, "not available on an allocation-free context");
+  const int64_t n = c->n, nt = c->ntile, n2 = n * n;
+  const int nB = c->nB;
+  if (n > 46340) return set_err(TLR_ERR_ARG, "the dense analysis is limited to N <= 46340");
+  hipStream_t st = c->stream;
+  HIPCHK(hipStreamSynchronize(st));
+  DevArr K, L, Ki, D, W, v;   // v: alpha, z, w, u, alpha_t (n each) and the eigenvalues (2 n)
+  HIPCHK(hipMalloc(&K.p, n2 * sizeof(double)));
+  HIPCHK(hipMalloc(&L.p, n2 * sizeof(double)));
+  HIPCHK(hipMalloc(&Ki.p, n2 * sizeof(double)));
+  HIPCHK(hipMalloc(&D.p, n2 * sizeof(double)));
+  HIPCHK(hipMalloc(&W.p, n2 * sizeof(double)));
+  HIPCHK(hipMalloc(&v.p, 7 * n * sizeof(double) + 64));
+  double *alpha = v.p, *z = v.p + n, *w = v.p + 2 * n, *u = v.p + 3 * n, *at = v.p + 4 * n, *eig = v.p + 5 * n,
+         *offd = v.p + 6 * n;
+  rocblas_int* dinfo = reinterpret_cast<rocblas_int*>(v.p + 7 * n);
+  Blas bl;
+  RBCHK(rocblas_create_handle(&bl.h));
+  RBCHK(rocblas_set_stream(bl.h, st));
+  rocblas_handle h = bl.h;
+  const rocblas_int ni = (rocblas_int)n;
+  rocblas_int info = 0;
+  // Sigma, its Cholesky factor, alpha, z, log L
+  MaternParams mp;
+  mp.sigma2 = theta[0];
+  mp.inv_range_sqrt3 = sqrt(3.0) / theta[1];
+  mp.nugget = theta[0] * theta[2];
+  HIPCHK(launch_dense_cov(c->sx, c->sy, n, mp, K.p, n, st));
+  HIPCHK(hipMemcpyAsync(L.p, K.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  RBCHK(rocsolver_dpotrf(h, rocblas_fill_lower, ni, L.p, ni, dinfo));
+  HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (info != 0) return set_err(TLR_ERR_ARG, "Sigma(theta) is not positive definite");
+  int rc = dense_loglik(c, h, L.p, alpha, &dense[0]);
+  if (rc < 0) return rc;
+  HIPCHK(hipMemcpyAsync(z, c->y, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  RBCHK(rocblas_dtrsv(h, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit, ni, L.p, ni, z, 1));
+  RBCHK(rocblas_ddot(h, ni, z, 1, z, 1, &dense[1]));
+  // Sigma^-1 (the lower triangle from potri, then mirrored)
+  HIPCHK(hipMemcpyAsync(Ki.p, L.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  RBCHK(rocsolver_dpotri(h, rocblas_fill_lower, ni, Ki.p, ni, dinfo));
+  HIPCHK(launch_symmetrize(Ki.p, n, n, 0, st));
+  std::vector<double> tmp(nt > 0 ? nt : 1);
+  if (kinv_tiles && nt > 0) {
+    HIPCHK(launch_tile_fro(Ki.p, n, n, c->b, c->bufR[0], nt, st));
+    HIPCHK(hipMemcpyAsync(kinv_tiles, c->bufR[0], nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  if (cov_tiles && nt > 0) {
+    HIPCHK(launch_tile_fro(K.p, n, n, c->b, c->bufR[0], nt, st));
+    HIPCHK(hipMemcpyAsync(cov_tiles, c->bufR[0], nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  if (alpha_blocks) {
+    std::vector<double> ah(n);
+    HIPCHK(hipMemcpyAsync(ah.data(), alpha, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < nB; ++i) {
+      double s = 0.0;
+      for (int64_t p = (int64_t)i * c->b; p < n && p < (int64_t)(i + 1) * c->b; ++p) s += ah[p] * ah[p];
+      alpha_blocks[i] = sqrt(s);
+    }
+  }
+  std::vector<double> err;
+  std::vector<int> rk;
+  std::vector<double> ev(n);
+  const double one = 1.0, zero = 0.0;
+  for (int q = 0; q < nrules; ++q) {
+    double* tq = terms + 6 * (int64_t)q;
+    rc = assembled_delta(c, theta, rules[q], values[q], D.p, err, rk);
+    if (rc < 0) return rc;
+    if (tile_err)
+      for (int64_t t = 0; t < nt; ++t) tile_err[(int64_t)q * nt + t] = err[t];
+    if (ranks) {
+      int32_t* d = ranks + (int64_t)q * nB * nB;
+      memset(d, 0, sizeof(int32_t) * nB * nB);
+      for (int i = 1; i < nB; ++i)
+        for (int j = 0; j < i; ++j) d[i * nB + j] = rk[tri_index(i, j)];
+    }
+    // log L of Sigma^TLR = Sigma + Delta (NaN when it is not positive definite)
+    RBCHK(rocblas_dgeam(h, rocblas_operation_none, rocblas_operation_none, ni, ni, &one, K.p, ni, &one, D.p, ni, W.p,
+                        ni));
+    RBCHK(rocsolver_dpotrf(h, rocblas_fill_lower, ni, W.p, ni, dinfo));
+    HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int e = 0; e < 6; ++e) tq[e] = NAN;
+    if (info != 0) continue;
+    rc = dense_loglik(c, h, W.p, at, &tq[0]);
+    if (rc < 0) return rc;
+    // first order: tr(Sigma^-1 Delta) (both symmetric: the sum of their entrywise product), alpha^T Delta alpha
+    rc = long_dot(h, n2, Ki.p, D.p, &tq[1]);
+    if (rc < 0) return rc;
+    RBCHK(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, D.p, ni, alpha, 1, &zero, w, 1));
+    RBCHK(rocblas_ddot(h, ni, alpha, 1, w, 1, &tq[2]));
+    // ||E z||^2 = (Delta alpha)^T Sigma^-1 (Delta alpha)
+    RBCHK(rocblas_dgemv(h, rocblas_operation_none, ni, ni, &one, Ki.p, ni, w, 1, &zero, u, 1));
+    RBCHK(rocblas_ddot(h, ni, w, 1, u, 1, &tq[5]));
+    // E = L^-1 Delta L^-T: two triangular solves with N right-hand sides
+    HIPCHK(hipMemcpyAsync(W.p, D.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    RBCHK(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
+                        ni, ni, &one, L.p, ni, W.p, ni));
+    RBCHK(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                        rocblas_diagonal_non_unit, ni, ni, &one, L.p, ni, W.p, ni));
+    double ef2 = 0.0;
+    rc = long_dot(h, n2, W.p, W.p, &ef2);
+    if (rc < 0) return rc;
+    tq[3] = sqrt(ef2);
+    // ||E||_2: the largest eigenvalue magnitude of its symmetric part
+    HIPCHK(launch_symmetrize(W.p, n, n, 1, st));
+    RBCHK(rocsolver_dsyevd(h, rocblas_evect_none, rocblas_fill_lower, ni, W.p, ni, eig, offd, dinfo));
+    HIPCHK(hipMemcpyAsync(ev.data(), eig, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (info != 0) return set_err(TLR_ERR_HIP, "symmetric eigensolver did not converge");
+    double e2 = 0.0;
+    for (int64_t i = 0; i < n; ++i) e2 = fmax(e2, fabs(ev[i]));
+    tq[4] = e2;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  return TLR_OK;
 }
 
 }  // extern "C"

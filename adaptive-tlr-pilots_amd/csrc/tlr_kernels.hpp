@@ -118,6 +118,16 @@ hipError_t launch_fill_int(int* p, int64_t count, int value, hipStream_t s);
 hipError_t launch_stash(double* U, double* V, int64_t fstride, int b, const int* r0, const int64_t* off,
                         double* stash, int* rank, int* rank0, int64_t ntile, int dir, hipStream_t s);
 hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t s);
+// dense analysis of the rule's tiles: Sigma(theta) (N x N) and Delta = Sigma^TLR - Sigma from the compressed
+// factors, with ||Delta_ij||_F^2 per tile
+hipError_t launch_dense_cov(const double* sx, const double* sy, int64_t n, MaternParams mp, double* K, int64_t ldk,
+                            hipStream_t s);
+hipError_t launch_assemble_delta(const double* sx, const double* sy, int64_t n, int b, const MaternParams* mp,
+                                 const double* U, const double* V, int64_t fstride, const int* rank, double* D,
+                                 int64_t ldd, double* err2, int64_t ntile, hipStream_t s);
+// ||A_ij||_F of the lower tiles of a dense matrix; its symmetric part in place (mode 0: lower -> upper, 1: mean)
+hipError_t launch_tile_fro(const double* A, int64_t lda, int64_t n, int b, double* out, int64_t ntile, hipStream_t s);
+hipError_t launch_symmetrize(double* A, int64_t lda, int64_t n, int mode, hipStream_t s);
 
 // factor.hip
 hipError_t launch_potrf(const SlotArgs& sa, int k, int nslot, hipStream_t s);

@@ -6,7 +6,8 @@ from .likelihood import (Lik, TLRContext, TLRFactor, crps_gauss, kl_tlr, morton_
                          rank_info, skill, tile_bounds, tlr_backward, tlr_forward, tlr_loglik, tlr_loglik_grad,
                          tlr_predict)
 from . import inference  # noqa: F401,E402  (MLE, MH, HMC and chain diagnostics over the likelihood)
+from .certificates import bound_terms  # noqa: F401,E402  (error terms of the assembled approximation)
 
-__all__ = ["Lik", "TLRContext", "TLRFactor", "TLRError", "crps_gauss", "device_info", "kl_tlr", "morton_order",
-           "neg_loglik_tlr", "rank_info", "skill", "tile_bounds", "tlr_backward", "tlr_forward", "tlr_loglik",
-           "tlr_loglik_grad", "tlr_predict"]
+__all__ = ["Lik", "TLRContext", "TLRFactor", "TLRError", "bound_terms", "crps_gauss", "device_info", "kl_tlr",
+           "morton_order", "neg_loglik_tlr", "rank_info", "skill", "tile_bounds", "tlr_backward", "tlr_forward",
+           "tlr_loglik", "tlr_loglik_grad", "tlr_predict"]

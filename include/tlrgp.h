@@ -54,9 +54,10 @@ extern "C" {
  *  TLR_PREC_FP64  plain FP64;
  *  TLR_PREC_FP32  FP32 throughout the recompression;
  *  TLR_PREC_BF16  the two products with bfloat16 operands and FP32 accumulation, the rest in FP32.
- * A pair whose budget is below `guard` (16384 by default, TLR_TUNE_GUARD) unit roundoffs of the working precision times the core's scale (the larger
- * of ||core||_F and ||U_ik W||_F ||U_jk||_F) is recompressed at the next higher precision instead (BF16 ->
- * FP32 -> FP64).  The frozen-rank replays of the gradient keep each pair's arithmetic.  The storage rule
+ * A pair whose budget is below `guard` (16384 by default, TLR_TUNE_GUARD) unit roundoffs of the working
+ * precision times the core's scale (the larger of ||core||_F and ||U_ik W||_F ||U_jk||_F) is recompressed at
+ * the next higher precision instead (BF16 -> FP32 -> FP64); the work lists apply the test first with an
+ * upper bound of that scale from per-tile norms and send the pairs that cannot pass straight to FP64.  The frozen-rank replays of the gradient keep each pair's arithmetic.  The storage rule
  * TLR_RULE_MIXED runs with TLR_PREC_FP64 only.  No counterpart in the reference (its arithmetic is FP64). */
 #define TLR_PREC_FP64 0
 #define TLR_PREC_FP32 1
@@ -184,6 +185,24 @@ int tlr_ctx_arith_counts(const tlr_ctx* ctx, int32_t slot, int64_t counts[3]);
 int tlr_loglik_noalloc(const double* S, const double* y, int64_t n, int32_t b, const double theta[3], int32_t rule,
                        double value, int32_t precision, double out3[3], int32_t* ranks_sigma, int32_t* ranks_factor,
                        int32_t* failed_at);
+
+/* The error terms of the assembled approximation Sigma^TLR (every off-diagonal tile replaced by the rule's
+ * rank-r truncation, codes/11_tlr_rank_rules.py:74-109) that codes/36_likelihood_error_terms.py:75-136
+ * decomposes, for nrules rules (rules[q] TLR_RULE_FIXED or TLR_RULE_ADAPTIVE, values[q] its value) at theta.
+ * The rule's tiles are the initial compression of an evaluation at theta (batch slot 0, whose resident factor
+ * is given up); Sigma, Delta = Sigma^TLR - Sigma and E = L^-1 Delta L^-T are dense N x N device arrays that
+ * this call allocates and frees (five of them: N <= 46340).  Outputs (host):
+ *   dense[2]             log L of Sigma, z^T z (z = L^-1 y);
+ *   kinv_tiles[n_tiles]  ||(Sigma^-1)_ij||_F, cov_tiles[n_tiles] ||Sigma_ij||_F (tri_index order; optional);
+ *   alpha_blocks[nB]     ||alpha_i|| of alpha = Sigma^-1 y per tile row (optional);
+ *   terms[6 nrules]      per rule: log L of Sigma^TLR (NaN when it is not positive definite, and then the
+ *                        rest NaN too), tr(Sigma^-1 Delta), alpha^T Delta alpha, ||E||_F, ||E||_2,
+ *                        (Delta alpha)^T Sigma^-1 (Delta alpha);
+ *   tile_err[nrules x n_tiles]  ||Delta_ij||_F (optional); ranks[nrules x nB x nB] as ranks_sigma (optional).
+ * TLR_ERR_ARG when Sigma itself is not positive definite. */
+int tlr_bound_terms(tlr_ctx* ctx, const double theta[3], int32_t nrules, const int32_t* rules, const double* values,
+                    double dense[2], double* kinv_tiles, double* cov_tiles, double* alpha_blocks, double* terms,
+                    double* tile_err, int32_t* ranks);
 
 /* Message of the last error on the calling thread. */
 const char* tlr_last_error(void);

@@ -258,6 +258,109 @@ def recompress(U, V, rule, thr_rel, thr_abs, rank=None):
     return Qu @ (Uw[:, :r] * s[:r]), Qv @ Vwt[:r].T, err
 
 
+def truncate_tiles(K, bounds, rule, value, m_theta):
+    """The assembled approximation Sigma^TLR in place: every off-diagonal tile of K (both triangles) replaced by
+    its rank-r truncation by the rule (fixed: relative Frobenius eps = value; adaptive: absolute budget
+    value * b * m_theta / N, b the largest tile).  Returns (stored values, max relative tile error, max
+    absolute tile error, sqrt(2 sum e_ij^2), ranks in tile order).  Follows 11:74-109."""
+    n = K.shape[0]
+    b = max(e - s for s, e in bounds)
+    stored = sum((e - s) ** 2 for s, e in bounds)
+    emax = value * b * m_theta / n
+    ranks, max_rel, max_abs, fro2 = [], 0.0, 0.0, 0.0
+    for i, (i0, i1) in enumerate(bounds):
+        for j, (j0, j1) in enumerate(bounds[:i]):
+            U, s_, Vt = _svd(K[i0:i1, j0:j1], False)
+            r, err = _rank_of(s_, "fixed" if rule == "fixed" else "adaptive", value, emax)
+            Tr = (U[:, :r] * s_[:r]) @ Vt[:r] if r > 0 else np.zeros((i1 - i0, j1 - j0))
+            K[i0:i1, j0:j1] = Tr
+            K[j0:j1, i0:i1] = Tr.T
+            stored += 2 * r * max(i1 - i0, j1 - j0)
+            ranks.append(r)
+            fro2 += 2 * err ** 2
+            max_abs = max(max_abs, err)
+            max_rel = max(max_rel, err / max(np.sqrt((s_ ** 2).sum()), 1e-300))
+    return stored, max_rel, max_abs, float(np.sqrt(fro2)), ranks
+
+
+def tile_fro(A, bounds):
+    """||A_ij||_F of the lower tiles i > j, keyed (i, j).  As 36:32-38."""
+    return {(i, j): float(np.linalg.norm(A[i0:i1, j0:j1]))
+            for i, (i0, i1) in enumerate(bounds) for j, (j0, j1) in enumerate(bounds[:i])}
+
+
+def bound_terms(S, y, theta, bounds, rules):
+    """The terms of the assembled approximation's log-likelihood error that 36 decomposes, for each (rule,
+    value) in rules: the error ell^TLR - ell (two dense Cholesky factorizations), the first-order term
+    T1 = -tr(Sigma^-1 Delta) / 2 + alpha^T Delta alpha / 2 and its tile-wise bounds, the remainder and its
+    bounds through E = L^-1 Delta L^-T (||E||_F, ||E||_2, ||E z||^2 = (Delta alpha)^T Sigma^-1 (Delta alpha)),
+    and the eta-route bounds.  Same keys as the reference's bound_terms JSON.  Follows 36:75-136 (the
+    spectral norm by a dense symmetric eigensolver instead of ARPACK's eigsh)."""
+    S = np.asarray(S, float)
+    y = np.asarray(y, float)
+    theta = np.asarray(theta, float)
+    n = len(y)
+    b = max(e - s for s, e in bounds)
+    m_theta = theta[0] * theta[2]
+    K = cov_full(S, theta)
+    c, low = cho_factor(K, lower=True, check_finite=False)
+    alpha = cho_solve((c, low), y, check_finite=False)
+    ll = -0.5 * (y @ alpha + 2.0 * np.log(np.diag(c)).sum() + n * LOG2PI)
+    z = solve_triangular(c, y, lower=True, check_finite=False)
+    z2 = float(z @ z)
+    Kinv = cho_solve((c, low), np.eye(n), check_finite=False)
+    L = np.tril(c)
+    Kinv_tiles = tile_fro(Kinv, bounds)
+    alpha_blk = [float(np.linalg.norm(alpha[i0:i1])) for (i0, i1) in bounds]
+    nb = len(bounds)
+    sumKinv = sum(Kinv_tiles.values())
+    sumalpha = sum(alpha_blk[i] * alpha_blk[j] for i in range(nb) for j in range(i))
+    out = dict(theta=theta.tolist(), m_theta=m_theta, dense_loglik=float(ll), z2=z2, sum_Kinv_tile_norms=sumKinv,
+               sum_alpha_block_products=sumalpha, rules={})
+    for rule, value in rules:
+        name = f"{rule}_{value:g}"
+        Kt = K.copy()
+        stored, max_rel, max_abs, fro, ranks = truncate_tiles(Kt, bounds, rule, value, m_theta)
+        emax = value * b * m_theta / n if rule == "adaptive" else None
+        llt, _ = loglik_dense(Kt, y)
+        if llt is None:
+            out["rules"][name] = dict(rule=rule, value=value, completed=False, stored_frac=stored / n ** 2,
+                                      max_abs_tile_err=max_abs, sqrt2sum_e2=fro)
+            continue
+        D = Kt - K
+        e_tiles = tile_fro(D, bounds)
+        tr_term = float((Kinv * D).sum())
+        q_term = float(alpha @ (D @ alpha))
+        T1 = -0.5 * tr_term + 0.5 * q_term
+        B_ld = sum(Kinv_tiles[k] * e_tiles[k] for k in e_tiles)
+        B_q = sum(alpha_blk[i] * alpha_blk[j] * e_tiles[(i, j)] for (i, j) in e_tiles)
+        E = solve_triangular(L, D, lower=True, check_finite=False)
+        E = solve_triangular(L, E.T, lower=True, check_finite=False)
+        E_F = float(np.linalg.norm(E))
+        E_2 = float(np.abs(np.linalg.eigvalsh(0.5 * (E + E.T))).max())
+        eta_hat = fro / m_theta
+        w = D @ alpha
+        Ez2 = float(w @ (Kinv @ w))
+        err = float(llt - ll)
+
+        def eta_route(eta):
+            return None if eta >= 1 else float(0.5 * (n * (-np.log1p(-eta)) + eta / (1 - eta) * z2))
+        out["rules"][name] = dict(
+            rule=rule, value=value, completed=True, stored_frac=stored / n ** 2, mean_rank=float(np.mean(ranks)),
+            max_rank=int(max(ranks)), e_max=emax, max_abs_tile_err=max_abs, max_rel_tile_err=max_rel,
+            sqrt2sum_e2=fro, eta_hat=eta_hat, error=err, first_order=T1, first_order_trace_part=-0.5 * tr_term,
+            first_order_quadratic_part=0.5 * q_term, remainder=err - T1, T1_bound_tilewise=B_ld + B_q,
+            T1_bound_tilewise_logdet=B_ld, T1_bound_tilewise_quadratic=B_q,
+            T1_bound_apriori_emax=None if emax is None else emax * (sumKinv + sumalpha), E_fro=E_F, E_2=E_2,
+            remainder_bound_logdet=E_F ** 2 / (4 * (1 - E_2)) if E_2 < 1 else None,
+            remainder_bound_quadratic=Ez2 / (2 * (1 - E_2)) if E_2 < 1 else None, Ez2=Ez2,
+            remainder_bound_quadratic_loose=E_2 ** 2 * z2 / (2 * (1 - E_2)) if E_2 < 1 else None,
+            remainder_bound_logdet_apriori=eta_hat ** 2 / (4 * (1 - eta_hat)) if eta_hat < 1 else None,
+            eta_route_bound_actual=eta_route(E_2), eta_route_bound_eta_hat=eta_route(eta_hat),
+            eta_route_bound_tau=eta_route(value) if rule == "adaptive" else None)
+    return out
+
+
 def rule_thresholds(rule, value, theta, n, b):
     """(thr_rel, thr_abs) of 17:149-150: fixed -> eps; adaptive and mixed -> tau * b * sigma2 * eta / N
     (b = max tile)."""

@@ -1,0 +1,67 @@
+"""Error terms of the assembled approximation (codes/36_likelihood_error_terms.py:75-136) on the GPU
+(tlrgp.bound_terms): against the reference's own results on its N=10k field for the four rules it ran
+(results/bound_terms/bound_terms_n10000.json, extracted to goldens.json), and against the oracle's
+restatement at N=2000."""
+import os
+
+import numpy as np
+import pytest
+
+import tlr_oracle as O
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+INT_KEYS = ("rule", "completed", "max_rank")
+
+
+@pytest.fixture(scope="module")
+def tl():
+    import tlrgp
+    return tlrgp
+
+
+def _compare(got, want, rtol, atol=0.0):
+    """Every key of want: equal for flags, ranks and names, relative rtol (absolute atol near zero) for floats."""
+    for k, wv in want.items():
+        gv = got.get(k)
+        if isinstance(wv, float) and wv is not None:
+            assert gv is not None and abs(gv - wv) <= rtol * abs(wv) + atol, (k, gv, wv)
+        else:
+            assert gv == wv, (k, gv, wv)
+
+
+def test_bound_terms_match_reference_n10000(tl):
+    import json
+    with open(os.path.join(GOLDEN, "goldens.json")) as f:
+        g = json.load(f)["bound_terms"]["10000"]
+    a = np.load(os.path.join(GOLDEN, "sim_n10000_obs.npy"))
+    S, y = a[:, :2].copy(), a[:, 2].copy()
+    rules = [(r["rule"], r["value"]) for r in g["rules"].values()]
+    print("case bound terms N=10000 b=250 rules", rules, flush=True)
+    out = tl.bound_terms(S, y, np.array(g["theta"]), tl.tile_bounds(10000, g["b"]), rules)
+    for k in ("dense_loglik", "z2", "sum_Kinv_tile_norms", "sum_alpha_block_products"):
+        assert abs(out[k] - g[k]) <= 1e-10 * abs(g[k]), (k, out[k], g[k])
+    for name, want in g["rules"].items():
+        got = out["rules"][name]
+        # the error terms are differences of log-likelihoods or sums over N^2 products of the two matrices:
+        # 1e-7 relative (measured <= 6e-9), and 1e-12 absolute for the small remainder
+        _compare(got, want, rtol=1e-7, atol=1e-12)
+        print(f"  {name}: error {got['error']:.10f} (reference {want['error']:.10f}), T1 {got['first_order']:.10f}, "
+              f"||E||_2 {got['E_2']:.10f}", flush=True)
+
+
+def test_bound_terms_match_oracle_n2000(tl):
+    a = np.load(os.path.join(GOLDEN, "sim_n2000_obs.npy"))
+    S, y = a[:, :2].copy(), a[:, 2].copy()
+    th = np.array([1.0, 0.3, 0.05])
+    bnd = tl.tile_bounds(2000, 250)
+    rules = [("adaptive", 0.9), ("fixed", 1e-4), ("fixed", 1e-2)]
+    print("case bound terms N=2000 b=250 rules", rules, flush=True)
+    got = tl.bound_terms(S, y, th, bnd, rules)
+    want = O.bound_terms(S, y, th, bnd, rules)
+    for k in ("dense_loglik", "z2", "sum_Kinv_tile_norms", "sum_alpha_block_products"):
+        assert abs(got[k] - want[k]) <= 1e-10 * abs(want[k]), (k, got[k], want[k])
+    for name, w in want["rules"].items():
+        print(f"  {name}: completed {got['rules'][name]['completed']} (oracle {w['completed']})", flush=True)
+        _compare(got["rules"][name], w, rtol=1e-7, atol=1e-12)

@@ -252,6 +252,59 @@ def test_dense_parts_match_bound_terms(goldens):
     assert abs(parts["loglik"] - g["dense_loglik"]) < 1e-9 * abs(g["dense_loglik"])
 
 
+@pytest.mark.slow
+def test_bound_terms_match_reference(goldens):
+    """The error terms of the assembled approximation (36:75-136 over 11:74-109) at N=10k for adaptive 0.9
+    against the reference's bound_terms_n10000.json: every term, bound and flag (measured within 1.5e-10
+    relative; ranks and stored values exact).  About a minute of dense work on 8 cores."""
+    S, y = _golden_field(10000)
+    g = goldens["bound_terms"]["10000"]
+    out = O.bound_terms(S, y, np.array(g["theta"]), O.tile_bounds(10000, g["b"]), [("adaptive", 0.9)])
+    for k in ("dense_loglik", "z2", "sum_Kinv_tile_norms", "sum_alpha_block_products"):
+        assert abs(out[k] - g[k]) <= 1e-10 * abs(g[k]), (k, out[k], g[k])
+    got, want = out["rules"]["adaptive_0.9"], g["rules"]["adaptive_0.9"]
+    for k, wv in want.items():
+        if isinstance(wv, float):
+            assert abs(got[k] - wv) <= 1e-8 * abs(wv), (k, got[k], wv)
+        else:
+            assert got[k] == wv, (k, got[k], wv)
+
+
+def test_truncate_tiles_closed_form():
+    """truncate_tiles on a matrix whose off-diagonal tiles have known singular values: the adaptive budget
+    keeps exactly the singular values above it, the fixed rule the smallest rank whose tail is within eps of
+    the tile norm; stored values and errors follow."""
+    rng = np.random.default_rng(3)
+    n, b = 60, 20
+    bnd = O.tile_bounds(n, b)
+    K = np.eye(n) * 10.0
+    svals = {}
+    for i, (i0, i1) in enumerate(bnd):
+        for j, (j0, j1) in enumerate(bnd[:i]):
+            Q1, _ = np.linalg.qr(rng.standard_normal((b, b)))
+            Q2, _ = np.linalg.qr(rng.standard_normal((b, b)))
+            s = np.array([1.0, 0.1, 0.01, 1e-3, 1e-4] + [1e-6] * (b - 5))
+            K[i0:i1, j0:j1] = (Q1 * s) @ Q2.T
+            K[j0:j1, i0:i1] = K[i0:i1, j0:j1].T
+            svals[(i, j)] = s
+    # adaptive: budget value * b * m / n = 2e-3; the tail after rank 2 is ~1.0e-2 > 2e-3, after rank 3
+    # ||(1e-3, 1e-4, 1e-6, ...)|| = sqrt(1e-6 + 1e-8 + (b - 5) 1e-12) ~ 1.005e-3 <= 2e-3: rank 3
+    Ka = K.copy()
+    stored, max_rel, max_abs, fro, ranks = O.truncate_tiles(Ka, bnd, "adaptive", 2e-3 * n / b, 1.0)
+    assert ranks == [3, 3, 3]
+    tail3 = np.sqrt(1e-6 + 1e-8 + (b - 5) * 1e-12)
+    assert abs(max_abs - tail3) < 1e-12 and abs(fro - np.sqrt(2 * 3) * tail3) < 1e-12
+    assert stored == 3 * b * b + 2 * 3 * 3 * b
+    # fixed eps = 1e-3: tail <= 1e-3 ||T||_F, ||T||_F = ||s||
+    Kf = K.copy()
+    _, _, _, _, ranks_f = O.truncate_tiles(Kf, bnd, "fixed", 1e-3, 1.0)
+    nrm = np.linalg.norm(svals[(1, 0)])
+    want = min(r for r in range(b + 1) if np.sqrt((svals[(1, 0)][r:] ** 2).sum()) <= 1e-3 * nrm)
+    assert ranks_f == [want] * 3
+    # the truncated tiles reproduce the rank-r sums exactly (to rounding), both triangles
+    np.testing.assert_allclose(Ka, Ka.T, atol=0)
+
+
 # ------------------------------------------------------------------ storage-precision rule (mixed)
 def test_round_to_closed_forms():
     """Representable values are fixed points; ties go to the even neighbour; E4M3 saturates at 448."""
