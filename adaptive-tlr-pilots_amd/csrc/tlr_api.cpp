@@ -42,7 +42,8 @@ static int set_err(int code, const std::string& msg) {
 
 namespace {
 
-constexpr int K1 = 64, K2 = 96;     // sketch widths of the two compression rounds
+constexpr int K1 = 48, K2 = 96;     // sketch widths of the two compression rounds (K1: default, tunable)
+constexpr int K_WARM = 64;          // smallest first round of a warm-started (replay) compression
 constexpr int PMIN = 24;            // spare sketch columns required to accept a rank
 constexpr int NPRODUCT = 4;         // products with T, T^T, T, T^T before the rank decision
 constexpr int NCS = 3;              // streams that compress slots concurrently
@@ -130,6 +131,7 @@ struct tlr_ctx {
   int recomp_grid_max[3][RCLASSES] = {};   // resident capacity (the grids above never exceed it)
   double guard = GUARD_DEFAULT;        // tunables (tlr_ctx_set_tunable)
   int diag_wg = 9;
+  int sketch1 = K1;                    // first-round sketch width (TLR_TUNE_SKETCH)
   int2* wlr = nullptr;                 // redo lists (panel x level) and their counters
   int* wlr_count = nullptr;
   int* rep_rpl = nullptr;              // max_batch records of recompression arithmetic (nlog each)
@@ -494,9 +496,12 @@ int tlr_ctx_set_y(tlr_ctx* ctx, const double* y) {
 // warm >= 0 (a replay whose ranks are given, forced != nullptr): the sketch starts from the right factors of
 // stash record `warm` (the recorded evaluation at a nearby theta), and two products (T X, T^T Q) replace the
 // four of a cold start.
-// cs: the compression stream (and workspace set) this slot uses.
+// cs: the compression stream (and workspace set) this slot uses.  k1: width of the first round (0: the
+// context's tunable; a warm start takes at least K_WARM columns: with two products instead of four its sketch
+// needs the wider margin, or the error left beside the truncation, smooth in theta, enters the gradient's
+// difference quotient).
 static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, int rule_fixed, const int* forced, int mixed,
-                         int warm, int cs) {
+                         int warm, int cs, int k1 = 0) {
   const MaternParams* mp = c->d_mp + slot;
   const double* thr2 = c->d_thr + 2 * slot;
   hipStream_t s = c->cls_stream[cs];
@@ -512,7 +517,8 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, int rule_fixed
   HIPCHK(launch_fill_int(need, nt, 1, s));
   const int64_t ms = (int64_t)b * c->kcap, rs = (int64_t)c->kcap * c->kcap;
   const int kmax = c->kcap;
-  const int widths[2] = {K1 < kmax ? K1 : kmax, kmax};
+  if (k1 <= 0) k1 = (warm >= 0 && forced && c->sketch1 < K_WARM) ? K_WARM : c->sketch1;
+  const int widths[2] = {k1 < kmax ? k1 : kmax, kmax};
   const int rounds = widths[0] == widths[1] ? 1 : 2;
   for (int rd = 0; rd < rounds; ++rd) {
     const int k = widths[rd];
@@ -1032,6 +1038,12 @@ int tlr_ctx_set_tunable(tlr_ctx* ctx, int32_t which, double value) {
           ctx->recomp_grid[pm][q] = g > 0 ? g : 1;
         }
       return TLR_OK;
+    case TLR_TUNE_SKETCH:
+      // a multiple of 8 (the register panels' column groups) with room for PMIN spare columns
+      if (!(value >= 32.0 && value <= (double)ctx->kcap) || (int)value != value || (int)value % 8 != 0)
+        return set_err(TLR_ERR_ARG, "sketch width must be a multiple of 8 in [32, min(96, b)]");
+      ctx->sketch1 = (int)value;
+      return TLR_OK;
     default:
       return set_err(TLR_ERR_ARG, "unknown tunable");
   }
@@ -1045,6 +1057,7 @@ int tlr_ctx_get_tunable(const tlr_ctx* ctx, int32_t which, double* value) {
     case TLR_TUNE_GRID_FRAC:
       *value = (double)ctx->recomp_grid[0][0] / (ctx->recomp_grid_max[0][0] > 0 ? ctx->recomp_grid_max[0][0] : 1);
       return TLR_OK;
+    case TLR_TUNE_SKETCH: *value = ctx->sketch1 < ctx->kcap ? ctx->sketch1 : ctx->kcap; return TLR_OK;
     default:
       return set_err(TLR_ERR_ARG, "unknown tunable");
   }
@@ -1315,7 +1328,9 @@ int assembled_delta(tlr_ctx* c, const double theta[3], int32_t rule, double valu
   HIPCHK(launch_fill_int(p.status + ST_FAILED, 1, 0x7fffffff, st));
   HIPCHK(launch_fill_int(c->d_retry, 1, 0, st));
   HIPCHK(hipMemsetAsync(D, 0, sizeof(double) * c->n * c->n, st));
-  int rc = compress_slot(c, 0, p, rule == TLR_RULE_FIXED, nullptr, 0, -1, 0);
+  // the error terms measure the truncation itself: the tiles come from a 64-column first round (a wider
+  // sketch leaves a smaller subspace defect beside the truncation than the likelihood's default)
+  int rc = compress_slot(c, 0, p, rule == TLR_RULE_FIXED, nullptr, 0, -1, 0, 64);
   if (rc < 0) {
     (void)hipStreamSynchronize(st);
     return rc;

@@ -1,6 +1,6 @@
 """Autotuner of a TLR context's tunables (no counterpart in the reference, which has no GPU path).
 
-Two kinds of tunables (TLRContext.set_tunable, C ABI tlr_ctx_set_tunable):
+Three kinds of tunables (TLRContext.set_tunable, C ABI tlr_ctx_set_tunable):
 
 * launch shape -- ``diag_wg`` (work groups per tile of the diagonal update) and ``grid_frac`` (persistent
   recompression grids as a fraction of the resident capacity).  They change which work group does what, not
@@ -10,6 +10,11 @@ Two kinds of tunables (TLRContext.set_tunable, C ABI tlr_ctx_set_tunable):
   recompression needs, else it is redone at the next higher precision.  A smaller guard sends more pairs to
   FP32 and moves log L further from plain FP64.  The tuner evaluates the sample theta in FP64 once and takes
   the smallest guard whose largest |log L - log L_FP64| stays within margin * bound with the same PD flags.
+* sketch width -- ``sketch``: columns of the first compression round.  A rank is accepted only with 24
+  sketch columns to spare (else the tile goes to the second round at min(96, b) columns), so the width
+  moves work between the rounds and changes the rounding of the compression, not its ranks.  The tuner
+  keeps the fastest width whose initial and final ranks equal the default width's on every sample theta and
+  whose log L agrees with it to a relative 1e-9.
 
 Usage: python -m tlrgp.autotune --field tests/golden/seed0_n40000.npy --b 500 --batch 20 --out tuning.json
 """
@@ -22,6 +27,7 @@ import numpy as np
 GUARDS = (256.0, 512.0, 1024.0, 2048.0, 4096.0, 16384.0, 1e6)
 DIAG_WGS = (3, 6, 9, 12, 18)
 GRID_FRACS = (0.5, 0.75, 1.0)
+SKETCHES = (64, 56, 48, 40, 32)
 
 
 def _timed(ctx, thetas, rule, value, precision, reps):
@@ -87,9 +93,41 @@ def tune_launch(ctx, thetas, rule="adaptive", value=0.9, precision="fp64", diag_
     return best, table
 
 
+def tune_sketch(ctx, thetas, rule="adaptive", value=0.9, precision="fp64", widths=SKETCHES, rtol=1e-9, reps=1,
+                log=print):
+    """Fastest first-round sketch width (the first of widths is the reference) whose initial and final ranks
+    equal the reference's on thetas, with the same PD flags and log L within rtol relative.  Leaves the
+    chosen width set; returns (width, table)."""
+    table, ref, times = [], None, []
+    for k in widths:
+        ctx.set_tunable("sketch", k)
+        best, r = None, None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            r = ctx.loglik_batch(thetas, rule, value, precision)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        if ref is None:
+            ref = r
+        ok = ref["completed"] & r["completed"]
+        same_ranks = bool(np.array_equal(ref["ranks_sigma"], r["ranks_sigma"]) and
+                          np.array_equal(ref["ranks_factor"][ok], r["ranks_factor"][ok]))
+        pd = bool(np.array_equal(ref["completed"], r["completed"]))
+        rel = float((np.abs(r["loglik"][ok] - ref["loglik"][ok]) / np.maximum(np.abs(ref["loglik"][ok]), 1.0)).max()) \
+            if ok.any() else 0.0
+        accepted = same_ranks and pd and rel <= rtol
+        table.append(dict(sketch=k, seconds=best, same_ranks=same_ranks, pd_agree=pd, max_rel_dlogL=rel,
+                          accepted=accepted))
+        times.append(best if accepted else np.inf)
+        log(f"autotune sketch {k}: {best:.3f} s, ranks equal {same_ranks}, PD agree {pd}, max rel dlogL {rel:.2e}")
+    chosen = widths[int(np.argmin(times))]
+    ctx.set_tunable("sketch", chosen)
+    return chosen, table
+
+
 def apply(ctx, tuning):
     """Set the tunables of a tuning dict (as written by main) on ctx."""
-    for name in ("guard", "diag_wg", "grid_frac"):
+    for name in ("guard", "diag_wg", "grid_frac", "sketch"):
         if name in tuning.get("chosen", {}):
             ctx.set_tunable(name, tuning["chosen"][name])
 
@@ -115,10 +153,11 @@ def main(argv=None):
     thetas = np.array([0.5, 0.1, 0.01]) + u * np.array([1.5, 0.5, 0.09])
     ctx = TLRContext(S, y, a.b, max_batch=a.batch)
     launch, t1 = tune_launch(ctx, thetas, a.rule, a.value, "fp64", reps=a.reps, log=lambda m: print(m, flush=True))
+    sketch, t3 = tune_sketch(ctx, thetas, a.rule, a.value, "fp64", reps=a.reps, log=lambda m: print(m, flush=True))
     guard, t2 = tune_guard(ctx, thetas, a.rule, a.value, a.precision, bound=a.bound, reps=a.reps,
                            log=lambda m: print(m, flush=True))
     out = dict(n=len(y), b=a.b, batch=a.batch, rule=a.rule, value=a.value, precision=a.precision, bound=a.bound,
-               chosen=dict(launch, guard=guard), launch_table=t1, guard_table=t2)
+               chosen=dict(launch, guard=guard, sketch=sketch), launch_table=t1, guard_table=t2, sketch_table=t3)
     print(json.dumps(out), flush=True)
     if a.out:
         with open(a.out, "w") as fh:

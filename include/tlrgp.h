@@ -159,10 +159,14 @@ int tlr_ctx_precisions(const tlr_ctx* ctx, int32_t slot, int32_t* prec);
  *  TLR_TUNE_GUARD      unit roundoffs of headroom a reduced-precision recompression needs (default 16384):
  *                      budget >= guard * u * scale, u = 2^-23 (FP32) or 2^-8 (BF16);
  *  TLR_TUNE_DIAG_WG    work groups per tile of the diagonal update (default 9);
- *  TLR_TUNE_GRID_FRAC  persistent recompression grids as a fraction (0, 1] of the resident capacity. */
+ *  TLR_TUNE_GRID_FRAC  persistent recompression grids as a fraction (0, 1] of the resident capacity;
+ *  TLR_TUNE_SKETCH     columns of the first compression round's sketch (default 48; a multiple of 8 from 32
+ *                      to the second round's width, min(96, b)): a rank is accepted only with 24 sketch
+ *                      columns to spare, so a narrower first round sends more tiles to the second. */
 #define TLR_TUNE_GUARD 0
 #define TLR_TUNE_DIAG_WG 1
 #define TLR_TUNE_GRID_FRAC 2
+#define TLR_TUNE_SKETCH 3
 int tlr_ctx_set_tunable(tlr_ctx* ctx, int32_t which, double value);
 int tlr_ctx_get_tunable(const tlr_ctx* ctx, int32_t which, double* value);
 
