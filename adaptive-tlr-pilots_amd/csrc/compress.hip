@@ -163,13 +163,19 @@ __global__ __launch_bounds__(64 * NW) void k_panel_qr(double* __restrict__ M, in
   __shared__ double tau[NW * CPW];
   RegPanel<RPL, NW, CPW> P;
   double* Mt = M + t * mstride;
+  // four columns per wave (the 33-64-column form): the reflector applications sum the wave's dot products
+  // together (0.5-0.8 % on the log L + gradient step at N = 40,000); with eight per wave the compiler puts
+  // the narrow forms' panel in scratch, so those keep one reduction per column
+  constexpr bool MV = CPW == 4;
   P.load(Mt, b, b, k);
-  P.geqr2(b, k, vbuf, tau);
+  if constexpr (MV) P.geqr2_mv(b, k, vbuf, tau);
+  else P.geqr2(b, k, vbuf, tau);
   if (R) {
     if (r_transposed) P.store_rt(R + t * rstride, k, k);
     else P.store_r(R + t * rstride, k, k);
   }
-  P.org2r(b, k, vbuf, tau);
+  if constexpr (MV) P.org2r_mv(b, k, vbuf, tau);
+  else P.org2r(b, k, vbuf, tau);
   P.store(Mt, b, b, k);
 }
 

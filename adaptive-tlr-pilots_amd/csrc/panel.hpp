@@ -30,6 +30,55 @@ __device__ __forceinline__ T cvt(float x) {
   else return (T)x;
 }
 
+// Sums over the 64 lanes of a wave of K = 4 or 8 values per lane; every lane gets all K totals.  A
+// reduce-scatter: each halving exchange (lanes l and l^1, l and l^2, and for K = 8 lanes l and l-4 within the
+// row of 16) leaves a lane half of the values it held, summed with its partner's copies, so that after two (K =
+// 4) or three (K = 8) of them each lane holds one value, the one selected by its lane bits 0-2; exchanges with
+// lanes 4 (K = 4) and 8 apart complete the row, two cross-row exchanges the wave, and one read per value
+// broadcasts the totals.  Every exchange within a row is a DPP move.  Compared with K separate wave_sum calls
+// this moves K + 4 values per lane instead of 4 K and reads 2 K lanes instead of 8 K; the additions run in
+// another order (the same sums, rounded differently).  Every lane of the wave must be active.
+template <int K>
+__device__ __forceinline__ void wave_sum_multi(double (&d)[K]) {
+  static_assert(K == 4 || K == 8, "4 or 8 values");
+  constexpr int H = K / 2, Q = K / 4;
+  const int l = lane_id();
+  const bool b0 = l & 1, b1 = (l >> 1) & 1, b2 = (l >> 2) & 1;
+  double e[H];
+#pragma unroll
+  for (int i = 0; i < H; ++i) {   // lanes with bit 0 set keep values H.., the others 0..H-1
+    const double send = b0 ? d[i] : d[i + H];
+    const double keep = b0 ? d[i + H] : d[i];
+    e[i] = keep + dpp_move<0xB1>(send);   // quad_perm(1, 0, 3, 2): lane l^1
+  }
+  double f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {   // bit 1 selects the upper half of what is left
+    const double send = b1 ? e[i] : e[i + Q];
+    const double keep = b1 ? e[i + Q] : e[i];
+    f[i] = keep + dpp_move<0x4E>(send);   // quad_perm(2, 3, 0, 1): lane l^2
+  }
+  double g;
+  if constexpr (K == 8) {   // bit 2 selects one of the two; lane l-4 (mod 16) has the other bit 2
+    const double send = b2 ? f[0] : f[1];
+    const double keep = b2 ? f[1] : f[0];
+    g = keep + dpp_move<0x124>(send);     // row_ror:4
+  } else {
+    g = f[0];
+    g += dpp_move<0x124>(g);              // lanes 4 apart hold the same value index
+  }
+  g += dpp_move<0x128>(g);                // row_ror:8: the row's 16 lanes
+  g += __shfl_xor(g, 16);
+  g += __shfl_xor(g, 32);
+  // value c ended in the lanes whose bits (0, 1[, 2]) spell c from its most significant bit down
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    const int lane = K == 8 ? (((c >> 2) & 1) | (((c >> 1) & 1) << 1) | ((c & 1) << 2))
+                            : (((c >> 1) & 1) | ((c & 1) << 1));
+    d[c] = readlane_d(g, lane);
+  }
+}
+
 template <int RPL, int NW, int CPW_ = 8, class T = double>
 struct RegPanel {
   static constexpr int CPW = CPW_;
@@ -143,6 +192,118 @@ struct RegPanel {
         }
         __syncthreads();
         apply_reflector(c, n, vb, tau[c]);
+      }
+    }
+    __syncthreads();
+  }
+
+  // apply_reflector with the wave's CPW dot products summed together (wave_sum_multi; T = double, CPW = 4 or
+  // 8).  Every wave must call it (the sums need all 64 lanes); a wave with no column left skips the work.
+  __device__ __forceinline__ void apply_reflector_mv(int c, int n, const T* vb, T tc) {
+    static_assert(sizeof(T) == 8 && (CPW == 4 || CPW == 8), "double panel, 4 or 8 columns per wave");
+    const int w = wave_id(), l = lane_id();
+    bool act = false;
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj) act |= (col_of(w, jj) > c && col_of(w, jj) < n);
+    if (!act) return;
+    T v[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) v[t] = vb[l + 64 * t];
+    T d[CPW];
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj) {
+      const int j = col_of(w, jj);
+      T s = (T)0;
+      if (j > c && j < n) {
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) s += v[t] * a[jj][t];
+      }
+      d[jj] = s;
+    }
+    wave_sum_multi<CPW>(d);
+#pragma unroll
+    for (int jj = 0; jj < CPW; ++jj) {
+      const int j = col_of(w, jj);
+      if (j > c && j < n) {
+        const T dd = d[jj] * tc;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) a[jj][t] -= dd * v[t];
+      }
+    }
+  }
+
+  // geqr2 with the reflector applications of apply_reflector_mv, and the diagonal entry read from its lane
+  // (c mod 64) instead of summed over the wave; same reflectors and R as geqr2 up to rounding.
+  __device__ __forceinline__ void geqr2_mv(int m, int n, T* vbuf, T* tau) {
+    const int w = wave_id(), l = lane_id();
+#pragma unroll
+    for (int jc = 0; jc < CPW; ++jc) {
+      for (int wo = 0; wo < NW; ++wo) {
+        const int c = wo + NW * jc;
+        if (c >= n) break;
+        T* vb = vbuf + (c & 1) * (64 * RPL);
+        if (w == wo) {
+          T part = (T)0, xc = a[jc][0];
+#pragma unroll
+          for (int t = 0; t < RPL; ++t) {
+            const int p = l + 64 * t;
+            const T x = a[jc][t];
+            if (p > c && p < m) part += x * x;
+            if (t == (c >> 6)) xc = x;
+          }
+          const T sigma = wave_sum(part);
+          const T alpha = readlane_d(xc, c & 63);
+          T beta, tc, scal;
+          if (sigma == (T)0) {
+            tc = (T)0; beta = alpha; scal = (T)0;
+          } else {
+            const T nrm = tsqrt(alpha * alpha + sigma);
+            beta = alpha >= (T)0 ? -nrm : nrm;
+            tc = (beta - alpha) / beta;
+            scal = (T)1 / (alpha - beta);
+          }
+#pragma unroll
+          for (int t = 0; t < RPL; ++t) {
+            const int p = l + 64 * t;
+            T v = (T)0;
+            if (p == c) v = (T)1;
+            else if (p > c && p < m) v = a[jc][t] * scal;
+            vb[p] = v;
+            if (p == c) a[jc][t] = beta;
+            else if (p > c) a[jc][t] = (p < m) ? v : (T)0;
+          }
+          if (l == 0) tau[c] = tc;
+        }
+        __syncthreads();
+        apply_reflector_mv(c, n, vb, tau[c]);
+      }
+    }
+    __syncthreads();
+  }
+
+  // org2r with the reflector applications of apply_reflector_mv.
+  __device__ __forceinline__ void org2r_mv(int m, int n, T* vbuf, const T* tau) {
+    const int w = wave_id(), l = lane_id();
+#pragma unroll
+    for (int jc = CPW - 1; jc >= 0; --jc) {
+      for (int wo = NW - 1; wo >= 0; --wo) {
+        const int c = wo + NW * jc;
+        if (c >= n) continue;
+        T* vb = vbuf + (c & 1) * (64 * RPL);
+        const T tc = tau[c];
+        if (w == wo) {
+#pragma unroll
+          for (int t = 0; t < RPL; ++t) {
+            const int p = l + 64 * t;
+            T v = (T)0;
+            if (p == c) v = (T)1;
+            else if (p > c && p < m) v = a[jc][t];
+            vb[p] = v;
+            a[jc][t] = (p == c) ? (T)1 - tc : ((p > c && p < m) ? -tc * v : (T)0);
+          }
+        }
+        __syncthreads();
+        apply_reflector_mv(c, n, vb, tc);
       }
     }
     __syncthreads();
