@@ -347,7 +347,7 @@ __global__ __launch_bounds__(256) void k_symmetrize(double* __restrict__ A, int6
 
 // ------------------------------------------------------------------------------ rank decision
 __global__ __launch_bounds__(256) void k_rank_decide(double* __restrict__ Rm, int64_t rstride,
-                                                     const double* __restrict__ Qrow, const double* __restrict__ Qcol,
+                                                     double* Qrow, const double* __restrict__ Qcol,
                                                      int64_t qstride, const double* __restrict__ part, int b, int k,
                                                      int kmax, int pmin, int rule_fixed,
                                                      const double* __restrict__ thr2,
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(256) void k_rank_decide(double* __restrict__ Rm, in
     }
     return;
   }
-  const double* Qr = Qrow + t * qstride;
+  double* Qr = Qrow + t * qstride;
   const double* Qc = Qcol + t * qstride;
   double* Ut = U + t * fstride;
   double* Vt = V + t * fstride;
@@ -442,10 +442,12 @@ __global__ __launch_bounds__(256) void k_rank_decide(double* __restrict__ Rm, in
     Vt[p + (int64_t)c * b] = v;
   }
   if (mixed) {
-    // storage precision under the budget left after truncation (17:163-164); the core R of this tile,
-    // already copied to LDS, gives the Gram workspace (ranks above sqrt(rstride / 3) stay FP64)
+    // storage precision under the budget left after truncation (17:163-164), for every pair as there; the
+    // tile's row basis (b x kcap doubles, read for the last time by the loop above) gives the Gram workspace:
+    // 3 r^2 doubles fit for every rank up to 64 when b >= 128 (only smaller tiles keep the ranks above
+    // sqrt(b kcap / 3) in FP64)
     __syncthreads();
-    if (3 * r * r <= rstride) round_pair_wg(Ut, b, b, Vt, b, b, r, thr_abs - tail[r], Rm + t * rstride, s_red);
+    if (3 * (int64_t)r * r <= qstride) round_pair_wg(Ut, b, b, Vt, b, b, r, thr_abs - tail[r], Qr, s_red);
   }
   if (tid == 0) {
     // ||U||_F of the stored factors (V orthonormal): the kept singular values
@@ -609,7 +611,7 @@ size_t rank_decide_lds(int k) {
   return (size_t)(2 * k * k + 2 * k + 1) * sizeof(double) + (size_t)k * sizeof(int);
 }
 
-hipError_t launch_rank_decide(double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
+hipError_t launch_rank_decide(double* Rm, int64_t rstride, double* Qrow, const double* Qcol,
                               int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
                               const double* thr2, double* U, double* V, int64_t fstride, int rank_cap,
                               int* rank, int* need, int* retry_count, int* status, const int* forced, int mixed,

@@ -572,7 +572,7 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, int rule_fixed
     on = c->tic(3, tq);
     HIPCHK(launch_panel_qr(M, ms, b, k, bufR, rs, last_op == 1 ? 1 : 0, need, nt, s));
     c->toc(on, tq);
-    const double* Qrow = (last_op == 0) ? M : Qprev;
+    double* Qrow = (last_op == 0) ? M : Qprev;   // also the mixed rule's Gram workspace
     const double* Qcol = (last_op == 0) ? Qprev : M;
     HIPCHK(launch_rank_decide(bufR, rs, Qrow, Qcol, ms, part, b, k, kmax, PMIN, rule_fixed, thr2,
                               p.U, p.V, (int64_t)b * RANK_CAP, RANK_CAP, p.rank, need, c->d_retry + slot,

@@ -107,7 +107,7 @@ hipError_t launch_residual(const double* sx, const double* sy, int64_t n, int b,
                            const double* A, const double* B, int64_t stride, int k, double* part, const int* active,
                            hipStream_t s);
 size_t rank_decide_lds(int k);
-hipError_t launch_rank_decide(double* Rm, int64_t rstride, const double* Qrow, const double* Qcol,
+hipError_t launch_rank_decide(double* Rm, int64_t rstride, double* Qrow, const double* Qcol,
                               int64_t qstride, const double* part, int b, int k, int kmax, int pmin, int rule_fixed,
                               const double* thr2, double* U, double* V, int64_t fstride, int rank_cap,
                               int* rank, int* need, int* retry_count, int* status, const int* forced, int mixed,

@@ -84,6 +84,24 @@ def test_mixed_n10000_vs_oracle_and_reference():
     assert abs(info["stored_factor_bytes"] / g["stored_factor_bytes"] - 1) < 5e-3
 
 
+def test_mixed_initial_rank_above_55_vs_oracle():
+    # b = 500, small nugget, tau = 0.2: the initial rank of tile (3, 2) is 58, which the second compression
+    # round (96 columns) accepts; its storage precision is chosen as for every other pair (17:163-164)
+    import tlrgp
+    import tlr_oracle as O
+    S, y = O.simulate_seed0(2000, seed=0)
+    th = np.array([1.0, 0.3, 0.001])
+    bnd = tlrgp.tile_bounds(2000, 500)
+    print("case mixed N=2000 (seed-0 field) b=500 tau=0.2 theta=(1, 0.3, 0.001)", flush=True)
+    ll, fac, info = tlrgp.tlr_loglik(S, y, th, bnd, "mixed", 0.2)
+    ll_o, _, info_o = O.tlr_loglik(S, y, th, bnd, "mixed", 0.2)
+    print("log L", ll, ll_o, "ranks", info["ranks_factor"].tolist(), info_o["ranks_factor"].tolist(),
+          "precisions", info["precision_factor"].tolist(), info_o["precision_factor"].tolist(), flush=True)
+    assert ll is not None and ll_o is not None
+    assert int(np.max(info_o["ranks_sigma"])) == 58
+    _statistical_match(info, info_o, ll, ll_o, 0.1)
+
+
 def test_mixed_gradient_refused():
     import tlrgp
     S, y = _field(2000)
