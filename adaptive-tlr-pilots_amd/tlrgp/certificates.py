@@ -12,8 +12,17 @@ The device work is one library call (tlr_bound_terms): Sigma, the rule's tiles (
 evaluation at theta: the randomized range finder with its FP64-verified rank decision, the reference's ranks)
 and Delta from this library's kernels; the dense factorizations, the inverse, the triangular solves for E and
 its spectral norm from rocSOLVER / rocBLAS in device memory (five N x N arrays: about 4 GB at N = 10,000).
-This module turns the returned quantities into the reference's bounds."""
+This module turns the returned quantities into the reference's bounds.
+
+Command line (the reference script's options and output file, on a field given as .npy with columns x, y, z in
+Morton order instead of one simulated by the script):
+  python -m tlrgp.certificates --field f.npy --b 250 --range 0.3 --eta 0.05 --rules adaptive:0.9,fixed:1e-4 --out dir
+"""
+import argparse
+import json
 import math
+import os
+import time
 
 import numpy as np
 
@@ -90,3 +99,44 @@ def bound_terms(S, y, theta, bounds, rules=DEFAULT_RULES):
             eta_route_bound_actual=eta_route(E_2), eta_route_bound_eta_hat=eta_route(eta_hat),
             eta_route_bound_tau=eta_route(value) if rule == "adaptive" else None)
     return out
+
+
+def main(argv=None):
+    """Write bound_terms_n{N}.json as the reference script does (36:40-52, 130-136): settings, the dense
+    terms and one entry per rule."""
+    from .likelihood import tile_bounds
+    ap = argparse.ArgumentParser(description="Error terms of the assembled TLR approximation (codes/36)")
+    ap.add_argument("--field", required=True, help=".npy with columns x, y, z (Morton-ordered sites, observations)")
+    ap.add_argument("--b", type=int, default=250)
+    ap.add_argument("--sigma2", type=float, default=1.0)
+    ap.add_argument("--range", type=float, default=0.3)
+    ap.add_argument("--eta", type=float, default=0.05)
+    ap.add_argument("--rules", type=str, default="adaptive:0.9,adaptive:0.5,fixed:1e-4,fixed:1e-5")
+    ap.add_argument("--out", type=str, default="bound_terms")
+    a = ap.parse_args(argv)
+    f = np.load(a.field)
+    S, y = f[:, :2].copy(), f[:, 2].copy()
+    theta = np.array([a.sigma2, a.range, a.eta])
+    rules = []
+    for spec in [x for x in a.rules.split(",") if x]:
+        rule, value = spec.split(":")
+        rules.append((rule, float(value)))
+    t0 = time.time()
+    out = bound_terms(S, y, theta, tile_bounds(len(y), a.b), rules)
+    out = dict(settings=dict(vars(a), n=len(y)), seconds=time.time() - t0, **out)
+    for name, r in out["rules"].items():
+        if r["completed"]:
+            print(f"{name}: stored {r['stored_frac']:.3f} | error {r['error']:+.4f} = T1 {r['first_order']:+.4f} + "
+                  f"remainder {r['remainder']:+.4f} | ||E||_2 {r['E_2']:.4f} ||E||_F {r['E_fro']:.4f}", flush=True)
+        else:
+            print(f"{name}: not positive definite (stored {r['stored_frac']:.3f})", flush=True)
+    os.makedirs(a.out, exist_ok=True)
+    path = os.path.join(a.out, f"bound_terms_n{len(y)}.json")
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print("saved " + path, flush=True)
+    return out
+
+
+if __name__ == "__main__":
+    main()

@@ -2,6 +2,7 @@
 (tlrgp.bound_terms): against the reference's own results on its N=10k field for the four rules it ran
 (results/bound_terms/bound_terms_n10000.json, extracted to goldens.json), and against the oracle's
 restatement at N=2000."""
+import json
 import os
 
 import numpy as np
@@ -32,7 +33,6 @@ def _compare(got, want, rtol, atol=0.0):
 
 
 def test_bound_terms_match_reference_n10000(tl):
-    import json
     with open(os.path.join(GOLDEN, "goldens.json")) as f:
         g = json.load(f)["bound_terms"]["10000"]
     a = np.load(os.path.join(GOLDEN, "sim_n10000_obs.npy"))
@@ -65,3 +65,18 @@ def test_bound_terms_match_oracle_n2000(tl):
     for name, w in want["rules"].items():
         print(f"  {name}: completed {got['rules'][name]['completed']} (oracle {w['completed']})", flush=True)
         _compare(got["rules"][name], w, rtol=1e-7, atol=1e-12)
+
+
+def test_bound_terms_command_line(tl, tmp_path):
+    from tlrgp import certificates
+    a = np.load(os.path.join(GOLDEN, "sim_n2000_obs.npy"))
+    f = tmp_path / "field.npy"
+    np.save(f, a)
+    print("case bound terms command line N=2000 b=250", flush=True)
+    out = certificates.main(["--field", str(f), "--b", "250", "--rules", "adaptive:0.9,fixed:1e-4",
+                             "--out", str(tmp_path)])
+    with open(tmp_path / "bound_terms_n2000.json") as fh:
+        saved = json.load(fh)
+    assert sorted(saved["rules"]) == ["adaptive_0.9", "fixed_0.0001"]
+    assert saved["settings"]["b"] == 250 and saved["settings"]["n"] == 2000
+    assert saved["dense_loglik"] == out["dense_loglik"] and saved["theta"] == [1.0, 0.3, 0.05]
