@@ -9,7 +9,9 @@ It restates, in NumPy/SciPy, the semantics of the reference scripts
 * ``codes/11_tlr_rank_rules.py``  -- kernel, distances, Morton order, tile bounds, dense log-likelihood,
   the data generator recipe;
 * ``codes/17_tlr_cholesky_pilot.py`` -- per-tile truncated SVD, QR+SVD recompression, the right-looking
-  TLR Cholesky, the tile forward/backward solves and the TLR log-likelihood.
+  TLR Cholesky, the tile forward/backward solves and the TLR log-likelihood;
+* ``codes/36_likelihood_error_terms.py`` -- the error terms of the assembled approximation (every
+  off-diagonal tile truncated by 11's ``truncate_tiles``) against the dense likelihood.
 
 The storage-precision ``mixed`` rule of 17 (adaptive ranks, then every factor pair stored in the lowest
 of FP8-E4M3 / FP16 / FP32 / FP64 whose rounding error fits the budget left after truncation, arithmetic
@@ -34,7 +36,9 @@ such generator) is pinned by its own definition: C^{-1} y gives back the PCG64 n
 sites, which are Morton-sorted, on fresh draws and on the committed N = 10k / 40k fields;
 ``cholesky_blocked`` against LAPACK's Cholesky, ``cov_full_lowmem`` entry for entry against ``cov_full``,
 ``tlr_matvec_L`` as the inverse of ``tlr_forward``, ``loglik_dense_parts`` against the reference's dense
-quadratic form (``bound_terms_n10000.json#z2``).
+quadratic form (``bound_terms_n10000.json#z2``).  ``truncate_tiles`` is pinned on tiles with known singular
+values (ranks, tile errors and stored values of both rules), ``bound_terms`` with ``tile_fro`` against every
+term the reference's ``bound_terms_n10000.json`` holds for the adaptive rule at 0.9.
 """
 import numpy as np
 from scipy.linalg import cho_factor, cho_solve, cholesky, solve_triangular, LinAlgError
