@@ -31,6 +31,8 @@ def main(argv=None):
                     help="a field's context holds chains x (1 + 4 / chunks) slots: the gradient's 4 replays per "
                          "chain run in this many batches")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--ckpt", default=None, help="state file: saved every --ckpt-every iterations, resumed from")
+    ap.add_argument("--ckpt-every", type=int, default=1)
     a = ap.parse_args(argv)
     rng = np.random.default_rng(a.seed)
     lo, hi = np.array([0.5, 0.1, 0.01]), np.array([2.0, 0.6, 0.1])
@@ -48,6 +50,7 @@ def main(argv=None):
     f = inference.concurrent_loglik_grad(parts)
     t0 = time.perf_counter()
     res = inference.hmc(f, np.concatenate(starts), a.iters, step=a.step, n_leapfrog=a.leapfrog, seed=a.seed,
+                        ckpt=a.ckpt, ckpt_every=a.ckpt_every,
                         log=lambda m: print(f"# {m} ({time.perf_counter() - t0:.0f} s)", flush=True))
     wall = time.perf_counter() - t0
     out = dict(fields=names, precision=a.precision, iters=a.iters, leapfrog=a.leapfrog, step=a.step,

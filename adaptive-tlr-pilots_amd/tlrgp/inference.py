@@ -387,7 +387,9 @@ def hmc(loglik_grad, starts, iters, step=0.01, n_leapfrog=10, seed=0, ckpt=None,
     the factorization fails; such proposals are rejected).  Each iteration runs n_leapfrog leapfrog steps of
     size `step` (one batched evaluation per step).  Returns dict with the chains (iters, M, 3), log L,
     acceptance per chain, evaluation count and per-leapfrog-step latencies.  log(message), when given, is
-    called after every iteration."""
+    called after every iteration.  ckpt: a file the state is saved to every ckpt_every iterations and resumed
+    from (the chains, the generator, the acceptance counts and the latencies of the evaluations so far, so
+    that a resumed run reports over all of them)."""
     rng = np.random.default_rng(seed)
     q = np.array(starts, float)
     M = q.shape[0]
@@ -404,6 +406,8 @@ def hmc(loglik_grad, starts, iters, step=0.01, n_leapfrog=10, seed=0, ckpt=None,
         q = ck["q"].copy()
         acc = ck["acc"].copy()
         rng.bit_generator.state = json.loads(str(ck["rng_state"]))
+        if "lat" in ck.files:
+            lat = [float(t) for t in ck["lat"]]
 
     def evaluate(x):
         t0 = time.perf_counter()
@@ -435,12 +439,13 @@ def hmc(loglik_grad, starts, iters, step=0.01, n_leapfrog=10, seed=0, ckpt=None,
         lls[it] = ll_q
         if ckpt and (it + 1) % ckpt_every == 0:
             _save_ckpt(ckpt, it=it + 1, chain=chain, lls=lls, q=q, acc=acc,
-                       rng_state=np.array(json.dumps(rng.bit_generator.state)))
+                       rng_state=np.array(json.dumps(rng.bit_generator.state)), lat=np.array(lat))
         if log:
             log(f"iteration {it + 1}/{iters}: {len(lat)} batched evaluations, last leapfrog step {lat[-1]:.3f} s, "
                 f"accepted {int(ok.sum())}/{M}")
     lat = np.array(lat)
-    return dict(chain=chain, lls=lls, acceptance=(acc / max(iters - it0, 1)).tolist(), evaluations=len(lat) * M,
+    # acc holds the acceptances of every iteration, the resumed ones included
+    return dict(chain=chain, lls=lls, acceptance=(acc / max(iters, 1)).tolist(), evaluations=len(lat) * M,
                 step_latency_p50=float(np.percentile(lat, 50)) if lat.size else None,
                 step_latency_p99=float(np.percentile(lat, 99)) if lat.size else None,
                 seconds=float(lat.sum()))

@@ -120,4 +120,8 @@ def test_hmc_batched_runs_and_resumes(tmp_path):
         I.hmc(interrupted, starts, 12, step=0.02, n_leapfrog=4, seed=5, ckpt=ck, ckpt_every=3)
     r2 = I.hmc(lg, starts, 12, step=0.02, n_leapfrog=4, seed=5, ckpt=ck, ckpt_every=3)
     assert np.array_equal(r["chain"], r2["chain"]) and np.array_equal(r["lls"], r2["lls"])
+    # acceptance over all 12 iterations; latencies of the 6 saved iterations carried over, plus the resumed
+    # run's own (its start is evaluated again)
+    assert r2["acceptance"] == r["acceptance"]
+    assert r2["evaluations"] == (1 + 6 * 4 + 1 + 6 * 4) * M
     ctx.close()
