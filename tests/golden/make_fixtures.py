@@ -7,7 +7,8 @@ Reads (text only, no code is run) the reference's ``results/`` folder:
   ``sim_n{N}_obs.npy`` (float64, columns x, y, z); the 300 held-out points of the N = 10k and 40k runs
   as ``sim_n{N}_holdout.npy``;
 * the paired result JSONs: dense and TLR log-likelihoods, stored values, mean/max factor ranks,
-  completion flags and failing tiles per rule, the TLR log-determinant of ``kl_tlr.json``, the dense
+  completion flags and failing tiles per rule, the TLR log-determinant of ``kl_tlr.json`` (the N = 10k
+  field and the three other Matern-3/2 fields of ``screening_stress/``, which share its sites), the dense
   quadratic form of ``bound_terms_n{N}.json`` and, at N = 10k, that file's error terms of every rule
   (codes/36_likelihood_error_terms.py), the Nelder-Mead MLE results (theta_hat, objective,
   evaluation and failure counts) of the N = 10k and 20k runs, the ``mixed`` storage-precision rule's
@@ -65,6 +66,20 @@ def main(ref="/root/reference"):
     g["kl_tlr_n10000"] = dict(logdet_exact=kl["logdet_exact"], logdet_approx=kl["logdet_approx"], kl=kl["kl"],
                               trace=kl["trace"], tau=kl["tau"],
                               note="screening field nu15_r03_e005 = the same N=10k field")
+    # the other Matern-3/2 screening fields: the KL certificate depends on the sites and theta only, and their
+    # sites are those of the N = 10k field
+    base = np.loadtxt(os.path.join(ref, "results", "tlr_cholesky", "sim_n10000_obs.csv"), delimiter=",",
+                      skiprows=1)[:, :2]
+    g["kl_tlr_screening"] = {}
+    for tag in ("nu15_r03_e05", "nu15_r1_e005", "nu15_r3_e005"):
+        st = _load_json(ref, f"screening_stress/{tag}/stress.json")["settings"]
+        sites = np.loadtxt(os.path.join(ref, "results", "screening_stress", tag, "sim_n10000_obs.csv"),
+                           delimiter=",", skiprows=1)[:, :2]
+        assert st["nu"] == 1.5 and st["n"] == 10000 and np.array_equal(sites, base), tag
+        kl = _load_json(ref, f"screening_stress/{tag}/kl_tlr.json")
+        g["kl_tlr_screening"][tag] = dict(theta=[st["sigma2"], st["range"], st["eta"]], b=st["b"],
+                                          logdet_exact=kl["logdet_exact"], logdet_approx=kl["logdet_approx"],
+                                          kl=kl["kl"], trace=kl["trace"], tau=kl["tau"])
     g["bound_terms"] = {}
     for n in (10000, 40000):
         bt = _load_json(ref, f"bound_terms/bound_terms_n{n}.json")

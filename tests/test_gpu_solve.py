@@ -1,5 +1,6 @@
 """Multi-right-hand-side solves with the GPU TLR factor (tlr_forward 17:230-240, tlr_backward 17:243-256)
-against the oracle, and the KL certificate of codes/31_kl_tlr.py against the reference's kl_tlr.json.
+against the oracle, and the KL certificate of codes/31_kl_tlr.py against the reference's kl_tlr.json (the N=10k
+field and the three other Matern-3/2 fields of its screening study).
 
 Tolerances: solves 1e-9 relative to the oracle's (the factors agree to rounding); the KL's trace and
 log-determinants 1e-9 relative, the KL itself (a difference 1e-5 the size of its terms) 1e-4 relative."""
@@ -50,3 +51,18 @@ def test_kl_tlr_matches_reference():
     for key in ("trace", "logdet_exact", "logdet_approx"):
         assert abs(r[key] - g[key]) <= 1e-9 * abs(g[key]), key
     assert abs(r["kl"] - g["kl"]) <= 1e-4 * abs(g["kl"])
+
+
+@pytest.mark.parametrize("tag", ["nu15_r03_e05", "nu15_r1_e005", "nu15_r3_e005"])
+def test_kl_tlr_screening_fields(tag):
+    # the other Matern-3/2 fields of the reference's screening study (same sites, larger nugget or range)
+    import tlrgp
+    with open(os.path.join(GOLDEN, "goldens.json")) as f:
+        g = json.load(f)["kl_tlr_screening"][tag]
+    S, _ = _field(10000)
+    print(f"case kl N=10000 b={g['b']} adaptive:{g['tau']} field {tag} theta={g['theta']}", flush=True)
+    r = tlrgp.kl_tlr(S, np.array(g["theta"]), tlrgp.tile_bounds(10000, g["b"]), "adaptive", g["tau"])
+    print(r, flush=True)
+    for key in ("trace", "logdet_exact", "logdet_approx"):
+        assert abs(r[key] - g[key]) <= 1e-9 * abs(g[key]), (key, r[key], g[key])
+    assert abs(r["kl"] - g["kl"]) <= 1e-4 * abs(g["kl"]), (r["kl"], g["kl"])
