@@ -12,7 +12,10 @@ Reads (text only, no code is run) the reference's ``results/`` folder:
   quadratic form of ``bound_terms_n{N}.json`` and, at N = 10k, that file's error terms of every rule
   (codes/36_likelihood_error_terms.py), the Nelder-Mead MLE results (theta_hat, objective,
   evaluation and failure counts) of the N = 10k and 20k runs, the ``mixed`` storage-precision rule's
-  results at N = 10k and 20k (``tlr_cholesky_mixed/``).  Written to ``goldens.json``;
+  results at N = 10k and 20k (``tlr_cholesky_mixed/``), and the initial-rank aggregates (mean and max
+  rank of the adaptive rule at 0.9 and the fixed rule at 1e-4) of every Matern-3/2 run of ``joint_sweep/``
+  (N = 10k, the same sites, b = 250 and 500, ranges 0.05-3, nuggets 0.05-1) whose ranks stay within 64.
+  Written to ``goldens.json``;
 * ``mcmc_tlrchol/<rule>/c<k>/``: the first 1200 draws of the reference's random-walk chains at N = 5000
   (adaptive 0.9 chains 1-4, fixed 1e-4 chains 1 and 4) with their settings, as ``mcmc_n5000_heads.npz``
   and the ``mcmc_n5000`` entry of ``goldens.json``.
@@ -80,6 +83,25 @@ def main(ref="/root/reference"):
         g["kl_tlr_screening"][tag] = dict(theta=[st["sigma2"], st["range"], st["eta"]], b=st["b"],
                                           logdet_exact=kl["logdet_exact"], logdet_approx=kl["logdet_approx"],
                                           kl=kl["kl"], trace=kl["trace"], tau=kl["tau"])
+    # initial ranks of the joint rank/precision sweep (codes/18_joint_rank_precision_pilot.py): rank_only is
+    # the adaptive rule's truncation, fixed the fixed rule's, both by full SVD of every off-diagonal tile
+    g["joint_sweep_ranks"] = []
+    seen = set()
+    for d in sorted(os.listdir(os.path.join(ref, "results", "joint_sweep"))):
+        j = _load_json(ref, f"joint_sweep/{d}/joint_rank_precision_n10000.json")
+        st = j["settings"]
+        key = (st["b"], st["range"], st["eta"])
+        if st.get("nu", 1.5) != 1.5 or key in seen:
+            continue
+        seen.add(key)
+        assert st["n"] == 10000 and st["seed"] == 20261020 and st["sigma2"] == 1.0, d
+        ra, rf = j["rules"]["rank_only"], j["rules"]["fixed"]
+        if max(ra["max_rank"], rf["max_rank"]) > 64:   # beyond the 64-column rank capacity of the GPU slots
+            continue
+        g["joint_sweep_ranks"].append(dict(run=d, b=st["b"], theta=[st["sigma2"], st["range"], st["eta"]],
+                                           tau=st["tau"], eps=st["eps"],
+                                           adaptive=dict(mean_rank=ra["mean_rank"], max_rank=ra["max_rank"]),
+                                           fixed=dict(mean_rank=rf["mean_rank"], max_rank=rf["max_rank"])))
     g["bound_terms"] = {}
     for n in (10000, 40000):
         bt = _load_json(ref, f"bound_terms/bound_terms_n{n}.json")

@@ -383,3 +383,26 @@ def test_frozen_replay_reproduces_and_differentiates():
     _, gt, _ = O.tlr_loglik_grad_fd(S, y, th, bnd, "fixed", 1e-9)
     _, gd = O.loglik_dense_grad(S, y, th)
     np.testing.assert_allclose(gt, gd, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("run", ["nu1.5_b500_eta0.05_r1.0", "b250_eta0.5_r0.3"])
+def test_truncate_svd_matches_joint_sweep_ranks(run):
+    # the initial ranks of the reference's joint rank/precision sweep (full SVD of every off-diagonal tile,
+    # codes/18_joint_rank_precision_pilot.py) at two of its settings: rank sums and largest ranks of both rules
+    with open(os.path.join(GOLDEN, "goldens.json")) as f:
+        e = next(x for x in json.load(f)["joint_sweep_ranks"] if x["run"] == run)
+    a = np.load(os.path.join(GOLDEN, "sim_n10000_obs.npy"))
+    S = a[:, :2]
+    th = np.array(e["theta"])
+    bnd = O.tile_bounds(10000, e["b"])
+    for rule, value in (("adaptive", e["tau"]), ("fixed", e["eps"])):
+        thr_rel, thr_abs = O.rule_thresholds(rule, value, th, 10000, e["b"])
+        ranks = []
+        for i, (i0, i1) in enumerate(bnd):
+            for j, (j0, j1) in enumerate(bnd[:i]):
+                T = th[0] * O.matern32(O.dist(S[i0:i1], S[j0:j1]), th[1])
+                ranks.append(O.truncate_svd(T, rule, thr_rel, thr_abs)[0].shape[1])
+        ranks = np.array(ranks)
+        assert int(ranks.sum()) == int(round(e[rule]["mean_rank"] * len(ranks))), (rule, ranks.sum())
+        assert int(ranks.max()) == e[rule]["max_rank"], (rule, ranks.max())
