@@ -14,8 +14,10 @@ Reads (text only, no code is run) the reference's ``results/`` folder:
   evaluation and failure counts) of the N = 10k and 20k runs, the ``mixed`` storage-precision rule's
   results at N = 10k and 20k (``tlr_cholesky_mixed/``), and the initial-rank aggregates (mean and max
   rank of the adaptive rule at 0.9 and the fixed rule at 1e-4) of every Matern-3/2 run of ``joint_sweep/``
-  (N = 10k, the same sites, b = 250 and 500, ranges 0.05-3, nuggets 0.05-1) whose ranks stay within 64.
-  Written to ``goldens.json``;
+  (N = 10k, the same sites, b = 250 and 500, ranges 0.05-3, nuggets 0.05-1) whose ranks stay within 64,
+  and for the four Matern-3/2 fields of ``screening_stress/`` the TLR maximum-likelihood point of
+  ``inference.json`` (theta_hat with the adaptive rule's log L and the dense log L there; the fields'
+  observations, on the N = 10k sites, as ``screening_<tag>_z.npy``).  Written to ``goldens.json``;
 * ``mcmc_tlrchol/<rule>/c<k>/``: the first 1200 draws of the reference's random-walk chains at N = 5000
   (adaptive 0.9 chains 1-4, fixed 1e-4 chains 1 and 4) with their settings, as ``mcmc_n5000_heads.npz``
   and the ``mcmc_n5000`` entry of ``goldens.json``.
@@ -83,6 +85,19 @@ def main(ref="/root/reference"):
         g["kl_tlr_screening"][tag] = dict(theta=[st["sigma2"], st["range"], st["eta"]], b=st["b"],
                                           logdet_exact=kl["logdet_exact"], logdet_approx=kl["logdet_approx"],
                                           kl=kl["kl"], trace=kl["trace"], tau=kl["tau"])
+    # the TLR MLE points of the same four fields (codes/27_inference_headtohead.py: tlr_loglik of 17 with the
+    # adaptive rule, and the dense log L, at the TLR theta_hat), with each field's observations
+    g["screening_tlr_mle"] = {}
+    for tag in ("nu15_r03_e005", "nu15_r03_e05", "nu15_r1_e005", "nu15_r3_e005"):
+        st = _load_json(ref, f"screening_stress/{tag}/stress.json")["settings"]
+        a = np.loadtxt(os.path.join(ref, "results", "screening_stress", tag, "sim_n10000_obs.csv"), delimiter=",",
+                       skiprows=1)
+        assert np.array_equal(a[:, :2], base), tag
+        np.save(os.path.join(HERE, f"screening_{tag}_z.npy"), a[:, 2].copy())
+        inf = _load_json(ref, f"screening_stress/{tag}/inference.json")
+        t = inf["tlr_mle"]
+        g["screening_tlr_mle"][tag] = dict(b=st["b"], tau=inf["tau"], theta_hat=t["theta_hat"],
+                                           tlr_loglik=t["tlr_loglik"], exact_loglik=t["exact_loglik"])
     # initial ranks of the joint rank/precision sweep (codes/18_joint_rank_precision_pilot.py): rank_only is
     # the adaptive rule's truncation, fixed the fixed rule's, both by full SVD of every off-diagonal tile
     g["joint_sweep_ranks"] = []

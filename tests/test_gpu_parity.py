@@ -194,3 +194,20 @@ def test_largest_reference_runs(tlrgp, goldens, n):
     for c in list(tlrgp.likelihood._CTX_CACHE.values()):
         c.close()
     tlrgp.likelihood._CTX_CACHE.clear()
+
+
+@pytest.mark.parametrize("tag", ["nu15_r03_e005", "nu15_r03_e05", "nu15_r1_e005", "nu15_r3_e005"])
+def test_screening_fields_at_tlr_mle(tlrgp, goldens, tag):
+    """The reference's TLR maximum-likelihood points on the four Matern-3/2 fields of its screening study
+    (codes/27_inference_headtohead.py: 17's log L with the adaptive rule at theta_hat): same N=10k sites,
+    each field's own observations, ranges 0.3-3, nuggets 0.03-0.44."""
+    g = goldens["screening_tlr_mle"][tag]
+    S, _ = _golden_field(10000)
+    y = np.load(os.path.join(GOLDEN, f"screening_{tag}_z.npy"))
+    th = np.array(g["theta_hat"])
+    print(f"case screening field {tag} N=10000 b={g['b']} adaptive:{g['tau']} theta_hat={g['theta_hat']}", flush=True)
+    ll, fac, info = tlrgp.tlr_loglik(S, y, th, tlrgp.tile_bounds(10000, g["b"]), "adaptive", g["tau"])
+    print("log L", ll, "reference", g["tlr_loglik"], flush=True)
+    assert info["completed"]
+    # the TLR error against the dense log L, as for the reference's own N=10k runs
+    assert _close(ll - g["exact_loglik"], g["tlr_loglik"] - g["exact_loglik"], rtol=1e-5), (ll, g["tlr_loglik"])

@@ -406,3 +406,16 @@ def test_truncate_svd_matches_joint_sweep_ranks(run):
         ranks = np.array(ranks)
         assert int(ranks.sum()) == int(round(e[rule]["mean_rank"] * len(ranks))), (rule, ranks.sum())
         assert int(ranks.max()) == e[rule]["max_rank"], (rule, ranks.max())
+
+
+@pytest.mark.slow
+def test_dense_loglik_at_screening_mle_points():
+    # the dense log L that the reference reports at its TLR maximum-likelihood points of the four Matern-3/2
+    # screening fields (inference.json#tlr_mle.exact_loglik; each field's observations on the N=10k sites)
+    with open(os.path.join(GOLDEN, "goldens.json")) as f:
+        gs = json.load(f)["screening_tlr_mle"]
+    S = np.load(os.path.join(GOLDEN, "sim_n10000_obs.npy"))[:, :2]
+    for tag, g in gs.items():
+        y = np.load(os.path.join(GOLDEN, f"screening_{tag}_z.npy"))
+        ll, _ = O.loglik_dense(O.cov_full(S, np.array(g["theta_hat"])), y)
+        assert abs(ll - g["exact_loglik"]) <= 1e-9 * abs(g["exact_loglik"]), (tag, ll, g["exact_loglik"])
