@@ -9,7 +9,7 @@ Reads (text only, no code is run) the reference's ``results/`` folder:
 * the paired result JSONs: dense and TLR log-likelihoods, stored values, mean/max factor ranks,
   completion flags and failing tiles per rule, the TLR log-determinant of ``kl_tlr.json`` (the N = 10k
   field and the three other Matern-3/2 fields of ``screening_stress/``, which share its sites), the dense
-  quadratic form of ``bound_terms_n{N}.json`` and, at N = 10k, that file's error terms of every rule
+  quadratic form of ``bound_terms_n{N}.json`` and that file's error terms of every rule (N = 10k, 40k)
   (codes/36_likelihood_error_terms.py), the Nelder-Mead MLE results (theta_hat, objective,
   evaluation and failure counts) of the N = 10k and 20k runs, the ``mixed`` storage-precision rule's
   results at N = 10k and 20k (``tlr_cholesky_mixed/``), and the initial-rank aggregates (mean and max
@@ -120,12 +120,11 @@ def main(ref="/root/reference"):
     g["bound_terms"] = {}
     for n in (10000, 40000):
         bt = _load_json(ref, f"bound_terms/bound_terms_n{n}.json")
-        g["bound_terms"][str(n)] = dict(dense_loglik=bt["dense_loglik"], z2=bt["z2"])
-        if n == 10000:   # the error terms of every rule (codes/36_likelihood_error_terms.py) at N = 10k
-            g["bound_terms"][str(n)].update(
-                theta=bt["theta"], b=bt["settings"]["b"], sum_Kinv_tile_norms=bt["sum_Kinv_tile_norms"],
-                sum_alpha_block_products=bt["sum_alpha_block_products"],
-                rules={k: {kk: vv for kk, vv in v.items() if kk != "seconds"} for k, v in bt["rules"].items()})
+        # the dense terms and the error terms of every rule (codes/36_likelihood_error_terms.py)
+        g["bound_terms"][str(n)] = dict(
+            dense_loglik=bt["dense_loglik"], z2=bt["z2"], theta=bt["theta"], b=bt["settings"]["b"],
+            sum_Kinv_tile_norms=bt["sum_Kinv_tile_norms"], sum_alpha_block_products=bt["sum_alpha_block_products"],
+            rules={k: {kk: vv for kk, vv in v.items() if kk != "seconds"} for k, v in bt["rules"].items()})
     g["mixed"] = {}   # the storage-precision rule mixed:0.9 (adaptive ranks + FP8/16/32/64 factor storage)
     for n in (10000, 20000):
         d = _load_json(ref, f"tlr_cholesky_mixed/rules_only/tlr_cholesky_n{n}.json") if n == 10000 else \
