@@ -118,16 +118,25 @@ class TLRContext:
         self.n = S.shape[0]
         self.b = int(b)
         self.max_batch = int(max_batch)
-        self._ptr = _lib.ctypes.c_void_p()
-        check(lib.tlr_ctx_create(_lib.ctypes.byref(self._ptr), dptr(S), dptr(y), self.n, self.b, self.max_batch))
+        self._handle = _lib.ctypes.c_void_p()
+        check(lib.tlr_ctx_create(_lib.ctypes.byref(self._handle), dptr(S), dptr(y), self.n, self.b, self.max_batch))
         nb = _lib.ctypes.c_int32()
         check(lib.tlr_ctx_dims(self._ptr, None, None, _lib.ctypes.byref(nb)))
         self.nB = nb.value
 
+    @property
+    def _ptr(self):
+        """The library's context handle; a closed context raises instead of passing a null handle on."""
+        h = self.__dict__.get("_handle")
+        if not h:
+            raise _lib.TLRError("the TLR context is closed")
+        return h
+
     def close(self):
-        if self._ptr:
-            lib.tlr_ctx_destroy(self._ptr)
-            self._ptr = _lib.ctypes.c_void_p()
+        h = self.__dict__.get("_handle")
+        if h:
+            lib.tlr_ctx_destroy(h)
+            self._handle = _lib.ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -281,16 +290,20 @@ _CTX_CACHE = {}
 
 def _context_for(S, y, b, max_batch=1):
     """Contexts are cached by the content of (S, y) and b: a sampler calling the likelihood repeatedly
-    on one field uploads it once."""
+    on one field uploads it once.  The cache keeps four; an evicted context is only dropped from it, so it
+    lives on (with its device memory) while a factor handle returned by tlr_loglik still refers to it."""
     S = f64(S)
     y = f64(y)
     h = hashlib.blake2b(S.tobytes(), digest_size=16)
     h.update(y.tobytes())
     key = (h.hexdigest(), int(b), S.shape[0])
     ctx = _CTX_CACHE.get(key)
+    if ctx is not None and not ctx.__dict__.get("_handle"):   # closed by its user (through a factor handle)
+        del _CTX_CACHE[key]
+        ctx = None
     if ctx is None or ctx.max_batch < max_batch:
         if len(_CTX_CACHE) >= 4:
-            _CTX_CACHE.pop(next(iter(_CTX_CACHE))).close()
+            _CTX_CACHE.pop(next(iter(_CTX_CACHE)))
         ctx = TLRContext(S, y, b, max_batch)
         _CTX_CACHE[key] = ctx
     return ctx

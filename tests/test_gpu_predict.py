@@ -86,3 +86,25 @@ def test_predict_reuses_resident_factor(tlrgp):
     z, o3 = ctx.solve(th, "adaptive", 0.9, y, "forward")
     assert ctx.factorizations() == n2
     assert abs(float(z @ z) - o3[2]) <= 1e-9 * o3[2]
+
+
+def test_factor_outlives_context_cache_eviction():
+    # tlr_loglik caches four contexts; a factor handle keeps its context alive after eviction, and a closed
+    # context refuses calls with an error instead of passing a null handle to the library
+    import tlrgp
+    a = np.load(os.path.join(GOLDEN, "sim_n2000_obs.npy"))
+    S, y = a[:, :2].copy(), a[:, 2].copy()
+    bnd = tlrgp.tile_bounds(2000, 250)
+    th = np.array([1.0, 0.3, 0.05])
+    Sp = S[:50] + 1e-3
+    print("case factor handle across context-cache eviction N=2000 b=250", flush=True)
+    ll0, fac0, _ = tlrgp.tlr_loglik(S, y, th, bnd, "adaptive", 0.9)
+    mu0, sd0 = tlrgp.tlr_predict(S, y, Sp, th, bnd, fac0)
+    for k in range(1, 6):   # five other fields (shifted observations): the first context leaves the cache
+        tlrgp.tlr_loglik(S, y + 0.01 * k, th, bnd, "adaptive", 0.9)
+    mu1, sd1 = tlrgp.tlr_predict(S, y, Sp, th, bnd, fac0)
+    np.testing.assert_array_equal(mu0, mu1)
+    np.testing.assert_array_equal(sd0, sd1)
+    fac0.ctx.close()
+    with pytest.raises(tlrgp.TLRError, match="closed"):
+        tlrgp.tlr_predict(S, y, Sp, th, bnd, fac0)
