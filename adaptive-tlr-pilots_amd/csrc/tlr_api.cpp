@@ -910,19 +910,22 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
     double res[4];
     HIPCHK(hipMemcpy(stv, p.status, sizeof(stv), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(res, p.result, sizeof(res), hipMemcpyDeviceToHost));
-    if (stv[ST_RANKCAP]) return set_err(TLR_ERR_RANKCAP, "tile rank exceeded the capacity of 64 columns");
     if (stv[ST_GUARD]) return set_err(TLR_ERR_HIP, "a kernel guard tripped (rank or core out of range)");
+    // a tile that needed more columns than the rank capacity: this theta has no result (the others of the
+    // batch do); a one-theta call reports it as an error
+    const bool capped = stv[ST_RANKCAP] != 0;
     const int fa = stv[ST_FAILED];
-    const bool ok = fa == 0x7fffffff;
+    const bool ok = !capped && fa == 0x7fffffff;
     out[3 * s + 0] = ok ? res[0] : NAN;
     out[3 * s + 1] = ok ? res[1] : NAN;
     out[3 * s + 2] = ok ? res[2] : NAN;
-    if (status) status[s] = ok ? TLR_OK : TLR_NOT_PD;
-    if (failed_at) failed_at[s] = ok ? -1 : fa;
-    if (!ok && m == 1) ret = TLR_NOT_PD;
+    if (status) status[s] = ok ? TLR_OK : (capped ? TLR_RANK_CAPPED : TLR_NOT_PD);
+    if (failed_at) failed_at[s] = ok || capped ? -1 : fa;
+    if (!ok && m == 1)
+      ret = capped ? set_err(TLR_ERR_RANKCAP, "tile rank exceeded the capacity of 64 columns") : TLR_NOT_PD;
     if (!replay) {   // a plain evaluation: its factor stays resident in the slot until the next evaluation
       SlotState& ss = c->sstate[s];
-      ss.valid = true;
+      ss.valid = !capped;
       memcpy(ss.theta, thetas + 3 * s, sizeof(ss.theta));
       ss.rule = rule;
       ss.value = value;

@@ -9,7 +9,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtlrgp.so")
 
-TLR_OK, TLR_NOT_PD = 0, 1
+TLR_OK, TLR_NOT_PD, TLR_RANK_CAPPED = 0, 1, 2
 TLR_ERR_ARG, TLR_ERR_HIP, TLR_ERR_RANKCAP, TLR_ERR_NODEVICE = -1, -2, -3, -4
 TLR_RULE_FIXED, TLR_RULE_ADAPTIVE, TLR_RULE_MIXED = 0, 1, 2
 RULES_C = {"fixed": TLR_RULE_FIXED, "adaptive": TLR_RULE_ADAPTIVE, "mixed": TLR_RULE_MIXED}

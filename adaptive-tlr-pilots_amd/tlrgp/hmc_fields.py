@@ -56,7 +56,8 @@ def main(argv=None):
     out = dict(fields=names, precision=a.precision, iters=a.iters, leapfrog=a.leapfrog, step=a.step,
                evaluations=res["evaluations"], evals_per_s=res["evaluations"] / res["seconds"],
                step_latency_p50=res["step_latency_p50"], step_latency_p99=res["step_latency_p99"],
-               wall_seconds=wall, acceptance_mean=float(np.mean(res["acceptance"])))
+               wall_seconds=wall, acceptance_mean=float(np.mean(res["acceptance"])),
+               rank_capped_evaluations=int(sum(getattr(fn, "rank_capped", 0) for fn, _ in parts)))
     print(json.dumps(out), flush=True)
     return out
 

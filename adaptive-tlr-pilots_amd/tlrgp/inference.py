@@ -275,21 +275,27 @@ def run_chain(lik, start, iters, burn, seed, ckpt=None, log=print, ckpt_every=10
 
 def batched_lik(ctx, rule, value, precision="fp64"):
     """thetas (k, 3) -> (log L (k,), ok (k,) bool) through one batched evaluation on a TLRContext
-    (k <= ctx.max_batch)."""
+    (k <= ctx.max_batch).  f.rank_capped counts the theta that had no result because a tile needed more than
+    the 64-column rank capacity (they count as failed factorizations)."""
     def f(thetas):
         r = ctx.loglik_batch(thetas, rule, value, precision, ranks=False)
+        f.rank_capped += int(r["rank_capped"].sum())
         ok = r["completed"].copy()
         return np.where(ok, r["loglik"], -np.inf), ok
+    f.rank_capped = 0
     return f
 
 
 def batched_loglik_grad(ctx, rule, value, h=1e-5, precision="fp64"):
     """log_thetas (k, 3) -> (log L, gradient in log theta) through one batched evaluation (-inf / NaN when
-    the factorization fails)."""
+    the factorization fails or a tile needs more than the 64-column rank capacity; f.rank_capped counts the
+    latter)."""
     def f(log_thetas):
         r = ctx.loglik_grad_batch(np.exp(log_thetas), rule, value, h, precision, ranks=False)
+        f.rank_capped += int(r["rank_capped"].sum())
         ok = r["completed"]
         return np.where(ok, r["loglik"], -np.inf), np.where(ok[:, None], r["grad"], np.nan)
+    f.rank_capped = 0
     return f
 
 

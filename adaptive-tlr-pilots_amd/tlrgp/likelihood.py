@@ -164,7 +164,9 @@ class TLRContext:
 
     def loglik_batch(self, thetas, rule, value, precision="fp64", ranks=True, graph=False):
         """Evaluate m <= max_batch theta in one pass.  Returns dict of arrays: loglik, logdet, quad (NaN when
-        not PD), completed (bool), failed_at (-1 when completed), ranks_sigma / ranks_factor (m, nB, nB).
+        not PD), completed (bool), failed_at (-1 when completed), ranks_sigma / ranks_factor (m, nB, nB),
+        rank_capped (bool: a tile of that theta needed more than 64 columns, so it has no result; with m > 1 the
+        others are evaluated, with m = 1 the call raises).
         graph: replay a HIP graph of this configuration (captured on its first use; same results)."""
         th = f64(np.atleast_2d(thetas))
         m = th.shape[0]
@@ -179,7 +181,7 @@ class TLRContext:
         check(fn(self._ptr, m, dptr(th), RULES[rule], float(value), PRECISIONS[precision], dptr(out), iptr(st),
                  iptr(fa), iptr(rs), iptr(rf)))
         res = dict(loglik=out[:, 0], logdet=out[:, 1], quad=out[:, 2], completed=st == _lib.TLR_OK,
-                   failed_at=fa, ranks_sigma=rs, ranks_factor=rf)
+                   failed_at=fa, ranks_sigma=rs, ranks_factor=rf, rank_capped=st == _lib.TLR_RANK_CAPPED)
         if rule == "mixed":
             res["precisions"] = np.stack([self.precisions(s) for s in range(m)])
         return res
@@ -236,7 +238,7 @@ class TLRContext:
         check(lib.tlr_loglik_grad_batch(self._ptr, m, dptr(th), RULES[rule], float(value), PRECISIONS[precision],
                                         float(h), dptr(out), dptr(grad), iptr(st), iptr(fa), iptr(rs), iptr(rf)))
         return dict(loglik=out[:, 0], logdet=out[:, 1], quad=out[:, 2], grad=grad, completed=st == _lib.TLR_OK,
-                    failed_at=fa, ranks_sigma=rs, ranks_factor=rf)
+                    failed_at=fa, ranks_sigma=rs, ranks_factor=rf, rank_capped=st == _lib.TLR_RANK_CAPPED)
 
     def loglik(self, theta, rule, value, precision="fp64"):
         """One theta: (ll | None, TLRFactor | None, info) as tlr_loglik of 17:259-266."""

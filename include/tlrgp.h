@@ -38,6 +38,10 @@ extern "C" {
 
 #define TLR_OK 0
 #define TLR_NOT_PD 1
+/* Per-theta status of the batch entry points (status[s]) when a tile of that theta needed more columns than
+ * the rank capacity of 64: its outputs are NaN and the batch's other theta are evaluated as usual.  A call
+ * with one theta returns TLR_ERR_RANKCAP instead.  No counterpart in the reference, which has no capacity. */
+#define TLR_RANK_CAPPED 2
 #define TLR_ERR_ARG (-1)
 #define TLR_ERR_HIP (-2)
 #define TLR_ERR_RANKCAP (-3)

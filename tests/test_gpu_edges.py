@@ -1,8 +1,13 @@
 """Edge shapes of the tile grid against the oracle: a short last tile, tile sizes that are not multiples of
 the 32/64-row blocks of the kernels, a two-tile problem, and both rank rules; initial and final ranks
-identical, log L, log-det and quadratic form within 1e-6 * max(1, |v|) (the reference checker's tolerance)."""
+identical, log L, log-det and quadratic form within 1e-6 * max(1, |v|) (the reference checker's tolerance).
+Also the rank capacity: a theta whose tile needs more than 64 columns fails alone in its batch."""
+import os
+
 import numpy as np
 import pytest
+
+from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
@@ -46,3 +51,26 @@ def test_edge_shapes_vs_oracle(n, b, theta, rule, value):
         assert _close(info["logdet"], info_o["logdet"]) and _close(info["quad"], info_o["quad"])
     else:
         assert info["failed_at"] == info_o["failed_at"]
+
+
+def test_rank_capacity_fails_only_its_theta():
+    # N=10k, b=500, range 0.05, nugget 0.05: a tile's initial rank is 66 (results/joint_sweep), beyond the
+    # 64-column capacity.  In a batch that theta alone has no result; alone it raises.
+    import tlrgp
+    a = np.load(os.path.join(GOLDEN, "sim_n10000_obs.npy"))
+    S, y = a[:, :2].copy(), a[:, 2].copy()
+    capped, fine = [1.0, 0.05, 0.05], [1.0, 0.3, 0.05]
+    ctx = tlrgp.TLRContext(S, y, 500, max_batch=10)
+    print("case rank capacity N=10000 b=500 adaptive:0.9", flush=True)
+    r = ctx.loglik_batch(np.array([capped, fine]), "adaptive", 0.9)
+    assert r["rank_capped"].tolist() == [True, False] and r["completed"].tolist() == [False, True]
+    assert np.isnan(r["loglik"][0]) and r["failed_at"][0] == -1
+    r1 = ctx.loglik_batch(np.array([fine]), "adaptive", 0.9)
+    np.testing.assert_allclose(r1["loglik"][0], r["loglik"][1], rtol=1e-12)
+    with pytest.raises(tlrgp.TLRError, match="capacity"):
+        ctx.loglik_batch(np.array([capped]), "adaptive", 0.9)
+    g = ctx.loglik_grad_batch(np.array([capped, fine]), "adaptive", 0.9)
+    g1 = ctx.loglik_grad_batch(np.array([fine]), "adaptive", 0.9)
+    assert g["rank_capped"].tolist() == [True, False] and np.all(np.isnan(g["grad"][0]))
+    np.testing.assert_allclose(g["grad"][1], g1["grad"][0], rtol=1e-9)
+    ctx.close()
