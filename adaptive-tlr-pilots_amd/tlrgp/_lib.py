@@ -69,12 +69,19 @@ lib = _load()
 
 
 class TLRError(RuntimeError):
-    pass
+    """An error of the library; code is its negative return code (TLR_ERR_*), None for the facade's own."""
+
+    def __init__(self, message, code=None):
+        super().__init__(message)
+        self.code = code
+
+
+TLR_ERR_ARG, TLR_ERR_HIP, TLR_ERR_RANKCAP, TLR_ERR_NODEVICE = -1, -2, -3, -4
 
 
 def check(rc):
     if rc < 0:
-        raise TLRError(f"tlrgp error {rc}: {lib.tlr_last_error().decode()}")
+        raise TLRError(f"tlrgp error {rc}: {lib.tlr_last_error().decode()}", rc)
     return rc
 
 

@@ -410,10 +410,17 @@ def kl_tlr(S, theta, bounds, rule, value):
 
 
 def neg_loglik_tlr(logth, S, y, bounds, rule, value, trace):
-    """Objective of the MLE: -ll, or 1e12 when the factorization fails (17:279-290; dense rule excluded)."""
+    """Objective of the MLE: -ll, or 1e12 when the factorization fails (17:279-290; dense rule excluded).
+    A theta with a tile beyond the 64-column rank capacity counts as a failure too (trace["rank_capped"])."""
     theta = np.exp(logth)
     trace["evals"] += 1
-    ll, _, _ = tlr_loglik(S, y, theta, bounds, rule, value)
+    try:
+        ll, _, _ = tlr_loglik(S, y, theta, bounds, rule, value)
+    except _lib.TLRError as e:
+        if e.code != _lib.TLR_ERR_RANKCAP:
+            raise
+        trace["rank_capped"] = trace.get("rank_capped", 0) + 1
+        ll = None
     if ll is None:
         trace["failures"] += 1
         return 1e12
@@ -421,7 +428,8 @@ def neg_loglik_tlr(logth, S, y, bounds, rule, value, trace):
 
 
 class Lik:
-    """Sampler-facing wrapper (38:39-56): lik(theta) -> ll, or -inf when the factorization fails."""
+    """Sampler-facing wrapper (38:39-56): lik(theta) -> ll, or -inf when the factorization fails (or a tile
+    of theta needs more than the 64-column rank capacity: last["rank_capped"])."""
 
     def __init__(self, S, y, bounds, rule, value, precision="fp64"):
         self.S, self.y, self.bounds, self.rule, self.value = f64(S), f64(y), bounds, rule, value
@@ -430,6 +438,13 @@ class Lik:
 
     def __call__(self, theta):
         t0 = time.time()
-        ll, fac, info = tlr_loglik(self.S, self.y, theta, self.bounds, self.rule, self.value, self.precision)
+        try:
+            ll, fac, info = tlr_loglik(self.S, self.y, theta, self.bounds, self.rule, self.value, self.precision)
+        except _lib.TLRError as e:
+            if e.code != _lib.TLR_ERR_RANKCAP:
+                raise
+            self.last = dict(ok=False, rank_capped=True, time=time.time() - t0, fac=None,
+                             info=dict(completed=False, rank_capped=True))
+            return -np.inf
         self.last = dict(ok=ll is not None, time=time.time() - t0, fac=fac, info=info)
         return -np.inf if ll is None else ll

@@ -74,3 +74,10 @@ def test_rank_capacity_fails_only_its_theta():
     assert g["rank_capped"].tolist() == [True, False] and np.all(np.isnan(g["grad"][0]))
     np.testing.assert_allclose(g["grad"][1], g1["grad"][0], rtol=1e-9)
     ctx.close()
+    # the one-theta samplers treat it as a failed factorization
+    bnd = tlrgp.tile_bounds(10000, 500)
+    lik = tlrgp.Lik(S, y, bnd, "adaptive", 0.9)
+    assert lik(np.array(capped)) == -np.inf and lik.last["rank_capped"]
+    trace = dict(evals=0, failures=0)
+    assert tlrgp.neg_loglik_tlr(np.log(capped), S, y, bnd, "adaptive", 0.9, trace) == 1e12
+    assert trace == dict(evals=1, failures=1, rank_capped=1)
