@@ -147,6 +147,27 @@ def test_hmc_gaussian_target_and_resume(tmp_path):
         I.hmc(lg_int, starts, 400, step=0.05, n_leapfrog=8, seed=11, ckpt=ck, ckpt_every=50)
     r2 = I.hmc(lg, starts, 400, step=0.05, n_leapfrog=8, seed=11, ckpt=ck, ckpt_every=50)
     assert np.array_equal(r["chain"], r2["chain"]) and np.array_equal(r["lls"], r2["lls"])
+    # acceptance over all 400 iterations; the latencies of the 150 checkpointed iterations are carried over
+    assert r2["acceptance"] == r["acceptance"]
+    assert r2["evaluations"] == (1 + 150 * 8 + 1 + 250 * 8) * 16
+
+
+def test_hmc_rejects_failed_evaluations():
+    # an evaluator that has no result in part of the space (NaN log L and gradient, as for a theta beyond the
+    # rank capacity or not positive definite): proposals landing there are rejected and the chains never
+    # hold such a point
+    mu = np.array([0.2, -1.2, -3.0])
+
+    def lg(x):
+        ll = -0.5 * (((x - mu) / 0.3) ** 2).sum(1)
+        g = -(x - mu) / 0.09
+        bad = x[:, 0] > 0.35
+        return np.where(bad, np.nan, ll), np.where(bad[:, None], np.nan, g)
+
+    starts = mu + np.random.default_rng(4).normal(0, 0.05, (8, 3))
+    r = I.hmc(lg, starts, 200, step=0.1, n_leapfrog=5, seed=2)
+    assert np.all(r["chain"][:, :, 0] <= 0.35) and np.all(np.isfinite(r["lls"]))
+    assert 0.1 < np.mean(r["acceptance"]) < 1.0
 
 
 def _ar1(phi, n, seed, m=1):
