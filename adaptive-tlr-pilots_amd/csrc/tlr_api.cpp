@@ -1378,8 +1378,9 @@ int long_dot(rocblas_handle h, int64_t m, const double* x, const double* y, doub
 }
 
 // log L = -(y^T alpha + 2 sum log diag L + N log 2 pi) / 2 from a Cholesky factor L (lower, in A) of Sigma;
-// alpha in the n-vector a
-int dense_loglik(tlr_ctx* c, rocblas_handle h, const double* A, double* a, double* ll) {
+// alpha in the n-vector a; the log-determinant and the quadratic form y^T alpha also when asked for
+int dense_loglik(tlr_ctx* c, rocblas_handle h, const double* A, double* a, double* ll, double* logdet = nullptr,
+                 double* quad = nullptr) {
   const int64_t n = c->n;
   hipStream_t st = c->stream;
   HIPCHK(hipMemcpyAsync(a, c->y, n * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -1394,10 +1395,52 @@ int dense_loglik(tlr_ctx* c, rocblas_handle h, const double* A, double* a, doubl
   double ld = 0.0;
   for (int64_t i = 0; i < n; ++i) ld += log(dg[i]);
   *ll = -0.5 * (ya + 2.0 * ld + (double)n * 1.8378770664093453);
+  if (logdet) *logdet = 2.0 * ld;
+  if (quad) *quad = ya;
   return TLR_OK;
 }
 
 }  // namespace
+
+extern "C" int tlr_dense_loglik(tlr_ctx* c, int32_t m, const double* thetas, double* out) {
+  if (!c || m < 0 || (m > 0 && (!thetas || !out))) return set_err(TLR_ERR_ARG, "invalid argument");
+  for (int s = 0; s < m; ++s) {
+    const double* th = thetas + 3 * s;
+    if (!(th[0] > 0.0) || !(th[1] > 0.0) || !(th[2] >= 0.0)) return set_err(TLR_ERR_ARG, "theta must be positive");
+  }
+  if (c->arena) return set_err(TLR_ERR_ARG, "not available on an allocation-free context");
+  const int64_t n = c->n, n2 = n * n;
+  if (n > 46340) return set_err(TLR_ERR_ARG, "the dense log-likelihood is limited to N <= 46340");
+  hipStream_t st = c->stream;
+  HIPCHK(hipStreamSynchronize(st));
+  DevArr K, v;   // Sigma and its factor; alpha (n) and the factorization's info word
+  HIPCHK(hipMalloc(&K.p, n2 * sizeof(double)));
+  HIPCHK(hipMalloc(&v.p, n * sizeof(double) + 64));
+  rocblas_int* dinfo = reinterpret_cast<rocblas_int*>(v.p + n);
+  Blas bl;
+  RBCHK(rocblas_create_handle(&bl.h));
+  RBCHK(rocblas_set_stream(bl.h, st));
+  for (int s = 0; s < m; ++s) {
+    const double* th = thetas + 3 * s;
+    MaternParams mp;
+    mp.sigma2 = th[0];
+    mp.inv_range_sqrt3 = sqrt(3.0) / th[1];
+    mp.nugget = th[0] * th[2];
+    HIPCHK(launch_dense_cov(c->sx, c->sy, n, mp, K.p, n, st));
+    RBCHK(rocsolver_dpotrf(bl.h, rocblas_fill_lower, (rocblas_int)n, K.p, (rocblas_int)n, dinfo));
+    rocblas_int info = 0;
+    HIPCHK(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double* o = out + 3 * (int64_t)s;
+    if (info != 0) {   // not positive definite
+      o[0] = o[1] = o[2] = NAN;
+      continue;
+    }
+    const int rc = dense_loglik(c, bl.h, K.p, v.p, &o[0], &o[1], &o[2]);
+    if (rc < 0) return rc;
+  }
+  return TLR_OK;
+}
 
 extern "C" {
 

@@ -49,6 +49,7 @@ SIGNATURES = {
     "tlr_loglik_noalloc": (ctypes.c_int, [_D, _D, ctypes.c_int64, ctypes.c_int32, _D, ctypes.c_int32, ctypes.c_double,
                                           ctypes.c_int32, _D, _I, _I, _I]),
     "tlr_bound_terms": (ctypes.c_int, [_VP, _D, ctypes.c_int32, _I, _D, _D, _D, _D, _D, _D, _D, _I]),
+    "tlr_dense_loglik": (ctypes.c_int, [_VP, ctypes.c_int32, _D, _D]),
     "tlr_last_error": (ctypes.c_char_p, []),
     "tlr_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32]),
 }

@@ -240,6 +240,18 @@ class TLRContext:
         return dict(loglik=out[:, 0], logdet=out[:, 1], quad=out[:, 2], grad=grad, completed=st == _lib.TLR_OK,
                     failed_at=fa, ranks_sigma=rs, ranks_factor=rf, rank_capped=st == _lib.TLR_RANK_CAPPED)
 
+    def dense_loglik_batch(self, thetas):
+        """The exact (dense FP64) log-likelihood at m theta, as loglik_dense of 11:112-120: dict of loglik,
+        logdet, quad (NaN when Sigma is not positive definite) and completed.  One N x N device array per call
+        (N <= 46340): the reference value beside the TLR evaluation, not its replacement."""
+        th = f64(np.atleast_2d(thetas))
+        m = th.shape[0]
+        if th.shape[1] != 3:
+            raise ValueError("thetas must be (m, 3)")
+        out = np.empty((m, 3))
+        check(lib.tlr_dense_loglik(self._ptr, m, dptr(th), dptr(out)))
+        return dict(loglik=out[:, 0], logdet=out[:, 1], quad=out[:, 2], completed=np.isfinite(out[:, 0]))
+
     def loglik(self, theta, rule, value, precision="fp64"):
         """One theta: (ll | None, TLRFactor | None, info) as tlr_loglik of 17:259-266."""
         r = self.loglik_batch(np.asarray(theta, float)[None, :], rule, value, precision)

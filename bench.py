@@ -46,6 +46,44 @@ STEP_TRAFFIC_PER_THETA = 0.435e9
 GRAD_H = {"fp64": 1e-5, "fp32": 1e-3, "bf16": 1e-3}
 
 
+def pd_sweep(ctx, m, batch, rule, value, seed=1):
+    """m theta drawn uniformly from the box (sigma2 in [0.5, 2], range in [0.1, 0.6], eta in [0.01, 0.1]): for
+    every off-diagonal arithmetic the PD rate, the log L error against the dense FP64 log L and |dlog L| against
+    plain FP64 TLR; and the dense log L's own PD rate."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.array([0.5, 0.1, 0.01]), np.array([2.0, 0.6, 0.1])
+    th = lo + rng.random((m, 3)) * (hi - lo)
+    dense = np.empty(m)
+    for i0 in range(0, m, 20):
+        d = ctx.dense_loglik_batch(th[i0:i0 + 20])
+        dense[i0:i0 + 20] = d["loglik"]
+        print(f"# pd sweep: dense log L {min(i0 + 20, m)}/{m}", file=sys.stderr, flush=True)
+    res = {}
+    for prec in ("fp64", "fp32", "bf16"):
+        ll = np.empty(m)
+        ok = np.empty(m, bool)
+        for i0 in range(0, m, batch):
+            r = ctx.loglik_batch(th[i0:i0 + batch], rule, value, precision=prec, ranks=False)
+            ll[i0:i0 + batch], ok[i0:i0 + batch] = r["loglik"], r["completed"]
+        print(f"# pd sweep: {prec} done", file=sys.stderr, flush=True)
+        res[prec] = (ll, ok)
+    ll64, ok64 = res["fp64"]
+    dok = np.isfinite(dense)
+    out = {"theta": m, "seed": seed, "box": {"sigma2": [0.5, 2.0], "range": [0.1, 0.6], "eta": [0.01, 0.1]},
+           "dense_pd_rate": float(dok.mean()), "per_precision": {}}
+    for prec, (ll, ok) in res.items():
+        both = ok & dok
+        err = np.abs(ll[both] - dense[both])
+        e = {"pd_rate": float(ok.mean()), "pd_agree_with_fp64": bool(np.array_equal(ok, ok64)),
+             "max_abs_err_vs_dense": float(err.max()) if err.size else None,
+             "median_abs_err_vs_dense": float(np.median(err)) if err.size else None}
+        if prec != "fp64":
+            b2 = ok & ok64
+            e["max_abs_dlogL_vs_fp64"] = float(np.abs(ll[b2] - ll64[b2]).max()) if b2.any() else None
+        out["per_precision"][prec] = e
+    return out
+
+
 def lhs_thetas(m, seed=0):
     from scipy.stats import qmc
     u = qmc.LatinHypercube(d=3, seed=seed).random(m)
@@ -170,6 +208,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=-1,
                     help="sites of the CPU-baseline sample (-1: the full N, one evaluation; 0: skip)")
     ap.add_argument("--no-grad", action="store_true", help="time log L alone")
+    ap.add_argument("--pd-sweep", type=int, default=0,
+                    help="theta drawn uniformly from the box for a PD-rate and error sweep per arithmetic against "
+                         "the dense FP64 log L (0: skip; 200 takes about 4 minutes at N = 40,000)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not supported (one GPU's share of the path is "
@@ -264,6 +305,8 @@ def main():
                                  "bytes_per_launch": pbytes / max(plaunches, 1),
                                  "achieved_gbs": pbytes / (pms * 1e-3) / 1e9 if pms > 0 else 0.0},
     }
+    if a.pd_sweep > 0:
+        out["pd_sweep"] = pd_sweep(ctx, a.pd_sweep, ctx.max_batch, a.rule, a.value)
     if a.cpu_sample:
         n_s = a.n if a.cpu_sample < 0 else a.cpu_sample
         out["cpu_baseline"] = cpu_baseline(S, y, a.b, tuple(thetas[0]), a.rule, a.value, n_s, 5 if grad else 1)

@@ -194,6 +194,13 @@ int tlr_loglik_noalloc(const double* S, const double* y, int64_t n, int32_t b, c
                        double value, int32_t precision, double out3[3], int32_t* ranks_sigma, int32_t* ranks_factor,
                        int32_t* failed_at);
 
+/* Dense FP64 log-likelihood of m theta: out[3 s + {0, 1, 2}] = log L, log-determinant, quadratic form of the
+ * full Sigma(theta) of the context's sites and observations (loglik_dense of codes/11_tlr_rank_rules.py:112-120,
+ * by a Cholesky factorization), NaN when Sigma is not positive definite.  The exact likelihood beside the TLR
+ * one, for the approximation's error; one N x N device array allocated and freed per call (N <= 46340),
+ * rocSOLVER's Cholesky. */
+int tlr_dense_loglik(tlr_ctx* ctx, int32_t m, const double* thetas, double* out);
+
 /* The error terms of the assembled approximation Sigma^TLR (every off-diagonal tile replaced by the rule's
  * rank-r truncation, codes/11_tlr_rank_rules.py:74-109) that codes/36_likelihood_error_terms.py:75-136
  * decomposes, for nrules rules (rules[q] TLR_RULE_FIXED or TLR_RULE_ADAPTIVE, values[q] its value) at theta.
