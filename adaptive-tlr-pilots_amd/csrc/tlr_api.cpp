@@ -45,7 +45,7 @@ namespace {
 constexpr int K1 = 48, K2 = 96;     // sketch widths of the two compression rounds (K1: default, tunable)
 constexpr int K_WARM = 64;          // smallest first round of a warm-started (replay) compression
 constexpr int PMIN = 24;            // spare sketch columns required to accept a rank
-constexpr int NPRODUCT = 4;         // products with T, T^T, T, T^T before the rank decision
+constexpr int NPRODUCT = 4;         // products with T, T^T, T, T^T before the rank decision (default, tunable)
 constexpr int NCS = 3;              // streams that compress slots concurrently
 constexpr int RANK_CAP = 64;        // column capacity of the U, V slots
 constexpr int WCAP = 128;           // widest recompression concatenation
@@ -132,6 +132,7 @@ struct tlr_ctx {
   double guard = GUARD_DEFAULT;        // tunables (tlr_ctx_set_tunable)
   int diag_wg = 9;
   int sketch1 = K1;                    // first-round sketch width (TLR_TUNE_SKETCH)
+  int nproduct = NPRODUCT;             // products of a cold compression (TLR_TUNE_PRODUCTS)
   int2* wlr = nullptr;                 // redo lists (panel x level) and their counters
   int* wlr_count = nullptr;
   int* rep_rpl = nullptr;              // max_batch records of recompression arithmetic (nlog each)
@@ -543,7 +544,7 @@ static int compress_slot(tlr_ctx* c, int slot, const SlotPtrs& p, int rule_fixed
       HIPCHK(launch_matern_product(c->sx, c->sy, c->n, b, nt, mp, 0, c->omega, 0, cur, ms, k, need, s));
     c->toc(on, tl);
     int last_op = 0;
-    const int nproduct = warm_start ? 2 : NPRODUCT;
+    const int nproduct = warm_start ? 2 : c->nproduct;
     for (int pidx = 1; pidx < nproduct; ++pidx) {
       double* src = (last_op == 0) ? cur : oth;
       double* dst = (last_op == 0) ? oth : cur;
@@ -1047,6 +1048,11 @@ int tlr_ctx_set_tunable(tlr_ctx* ctx, int32_t which, double value) {
         return set_err(TLR_ERR_ARG, "sketch width must be a multiple of 8 in [32, min(96, b)]");
       ctx->sketch1 = (int)value;
       return TLR_OK;
+    case TLR_TUNE_PRODUCTS:
+      if (!(value >= 2.0 && value <= 6.0) || (int)value != value)
+        return set_err(TLR_ERR_ARG, "products must be an integer in [2, 6]");
+      ctx->nproduct = (int)value;
+      return TLR_OK;
     default:
       return set_err(TLR_ERR_ARG, "unknown tunable");
   }
@@ -1061,6 +1067,7 @@ int tlr_ctx_get_tunable(const tlr_ctx* ctx, int32_t which, double* value) {
       *value = (double)ctx->recomp_grid[0][0] / (ctx->recomp_grid_max[0][0] > 0 ? ctx->recomp_grid_max[0][0] : 1);
       return TLR_OK;
     case TLR_TUNE_SKETCH: *value = ctx->sketch1 < ctx->kcap ? ctx->sketch1 : ctx->kcap; return TLR_OK;
+    case TLR_TUNE_PRODUCTS: *value = ctx->nproduct; return TLR_OK;
     default:
       return set_err(TLR_ERR_ARG, "unknown tunable");
   }

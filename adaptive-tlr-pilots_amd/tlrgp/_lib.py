@@ -14,7 +14,7 @@ TLR_ERR_ARG, TLR_ERR_HIP, TLR_ERR_RANKCAP, TLR_ERR_NODEVICE = -1, -2, -3, -4
 TLR_RULE_FIXED, TLR_RULE_ADAPTIVE, TLR_RULE_MIXED = 0, 1, 2
 RULES_C = {"fixed": TLR_RULE_FIXED, "adaptive": TLR_RULE_ADAPTIVE, "mixed": TLR_RULE_MIXED}
 TLR_PREC_FP64, TLR_PREC_FP32, TLR_PREC_BF16 = 0, 1, 2
-TLR_TUNE_GUARD, TLR_TUNE_DIAG_WG, TLR_TUNE_GRID_FRAC, TLR_TUNE_SKETCH = 0, 1, 2, 3
+TLR_TUNE_GUARD, TLR_TUNE_DIAG_WG, TLR_TUNE_GRID_FRAC, TLR_TUNE_SKETCH, TLR_TUNE_PRODUCTS = 0, 1, 2, 3, 4
 
 # every entry point of include/tlrgp.h with its ctypes signature
 _D = ctypes.POINTER(ctypes.c_double)

@@ -187,13 +187,13 @@ class TLRContext:
         return res
 
     TUNABLES = {"guard": _lib.TLR_TUNE_GUARD, "diag_wg": _lib.TLR_TUNE_DIAG_WG, "grid_frac": _lib.TLR_TUNE_GRID_FRAC,
-                "sketch": _lib.TLR_TUNE_SKETCH}
+                "sketch": _lib.TLR_TUNE_SKETCH, "products": _lib.TLR_TUNE_PRODUCTS}
 
     def set_tunable(self, name, value):
         """Set a launch / precision tunable: guard (headroom of the reduced-precision recompressions, in unit
         roundoffs), diag_wg (work groups per tile of the diagonal update), grid_frac (persistent recompression
         grids as a fraction of the resident capacity), sketch (columns of the first compression round, a
-        multiple of 8 in [32, min(96, b)])."""
+        multiple of 8 in [32, min(96, b)]), products (products with T and T^T of a cold compression, 2-6)."""
         check(lib.tlr_ctx_set_tunable(self._ptr, self.TUNABLES[name], float(value)))
 
     def get_tunable(self, name):

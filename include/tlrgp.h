@@ -171,6 +171,9 @@ int tlr_ctx_precisions(const tlr_ctx* ctx, int32_t slot, int32_t* prec);
 #define TLR_TUNE_DIAG_WG 1
 #define TLR_TUNE_GRID_FRAC 2
 #define TLR_TUNE_SKETCH 3
+/*  TLR_TUNE_PRODUCTS   products with T and T^T of a cold compression before its rank decision (default 4: two
+ *                      power iterations; an integer from 2 to 6). */
+#define TLR_TUNE_PRODUCTS 4
 int tlr_ctx_set_tunable(tlr_ctx* ctx, int32_t which, double value);
 int tlr_ctx_get_tunable(const tlr_ctx* ctx, int32_t which, double* value);
 

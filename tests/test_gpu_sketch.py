@@ -54,3 +54,24 @@ def test_sketch_width_rejected_outside_its_range(tl):
         with pytest.raises(tl.TLRError):
             ctx.set_tunable("sketch", bad)
     assert ctx.get_tunable("sketch") == 48
+
+
+def test_products_tunable():
+    # three products instead of four (one power iteration fewer) keep the ranks at N=2000 and move log L only by
+    # the larger subspace defect beside the truncation; the range is enforced
+    import tlr_oracle as O
+    import tlrgp
+    S, y = O.simulate_seed0(2000, seed=0)
+    ctx = tlrgp.TLRContext(S, y, 250, max_batch=len(THETAS))
+    assert ctx.get_tunable("products") == 4
+    print("case products 4 vs 3 N=2000 b=250", flush=True)
+    r4 = ctx.loglik_batch(THETAS, "adaptive", 0.9)
+    ctx.set_tunable("products", 3)
+    r3 = ctx.loglik_batch(THETAS, "adaptive", 0.9)
+    np.testing.assert_array_equal(r3["ranks_sigma"], r4["ranks_sigma"])
+    np.testing.assert_allclose(r3["loglik"], r4["loglik"], rtol=1e-6)
+    for bad in (1, 7, 3.5):
+        with pytest.raises(tlrgp.TLRError):
+            ctx.set_tunable("products", bad)
+    assert ctx.get_tunable("products") == 3
+    ctx.close()
