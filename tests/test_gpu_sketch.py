@@ -59,7 +59,6 @@ def test_sketch_width_rejected_outside_its_range(tl):
 def test_products_tunable():
     # three products instead of four (one power iteration fewer) keep the ranks at N=2000 and move log L only by
     # the larger subspace defect beside the truncation; the range is enforced
-    import tlr_oracle as O
     import tlrgp
     S, y = O.simulate_seed0(2000, seed=0)
     ctx = tlrgp.TLRContext(S, y, 250, max_batch=len(THETAS))
