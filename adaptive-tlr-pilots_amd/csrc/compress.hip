@@ -430,16 +430,37 @@ __global__ __launch_bounds__(256) void k_rank_decide(double* __restrict__ Rm, in
   const double* Qc = Qcol + t * qstride;
   double* Ut = U + t * fstride;
   double* Vt = V + t * fstride;
-  for (int e = tid; e < b * r; e += blockDim.x) {
-    const int p = e % b, c = e / b;
-    const int pc = perm[c];
-    double u = 0.0, v = 0.0;
-    for (int l = 0; l < k; ++l) {
-      u = fma(Qr[p + (int64_t)l * b], G[l + pc * k], u);
-      v = fma(Qc[p + (int64_t)l * b], J[l + pc * k], v);
+  // U = Qrow G[:, perm[:r]], V = Qcol J[:, perm[:r]]: a thread owns a row p and keeps CB output columns in
+  // registers, so each Q panel is read once per CB columns (not once per column); the sum over l runs in the
+  // same order as a column-by-column product
+  constexpr int CB = 8;
+  for (int c0 = 0; c0 < r; c0 += CB) {
+    const int nc = min(CB, r - c0);   // uniform over the work group
+    int pc[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) pc[j] = perm[c0 + (j < nc ? j : 0)];
+    for (int p = tid; p < b; p += blockDim.x) {
+      double u[CB], v[CB];
+#pragma unroll
+      for (int j = 0; j < CB; ++j) u[j] = v[j] = 0.0;
+      for (int l = 0; l < k; ++l) {
+        const double qr = Qr[p + (int64_t)l * b], qc = Qc[p + (int64_t)l * b];
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+          if (j < nc) {
+            u[j] = fma(qr, G[l + pc[j] * k], u[j]);
+            v[j] = fma(qc, J[l + pc[j] * k], v[j]);
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < CB; ++j) {
+        if (j < nc) {
+          Ut[p + (int64_t)(c0 + j) * b] = u[j];
+          Vt[p + (int64_t)(c0 + j) * b] = v[j];
+        }
+      }
     }
-    Ut[p + (int64_t)c * b] = u;
-    Vt[p + (int64_t)c * b] = v;
   }
   if (mixed) {
     // storage precision under the budget left after truncation (17:163-164), for every pair as there; the
