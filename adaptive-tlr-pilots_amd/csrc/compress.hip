@@ -49,12 +49,15 @@ __global__ void k_gen_diag(const double* __restrict__ sx, const double* __restri
 //
 // On the matrix cores (v_mfma_f64_16x16x4f64), computing out^T = X^T op(T)^T so that the accumulators hold
 // 16 consecutive rows of one output column per 16 lanes (stores in 128-byte runs).  Work group: 4 waves,
-// each 32 rows x 64 columns (2 x 4 accumulator blocks of 16 x 16); the work group covers 128 rows.  Per step
-// of 4 inner indices a lane generates the 2 kernel entries of its operand (rows p0 + lane % 16 and
-// p0 + 16 + lane % 16, inner index lane / 16) and reads 4 operands of X from LDS; the 8 MFMAs reuse each
-// generated entry for 64 columns, and the entry generation (VALU) overlaps the matrix pipe.  On 3160-tile
-// tests its outputs equal those of a sequential FMA loop over the inner index bit for bit.
-constexpr int MM_ROWS = 128, MM_COLS = 64, MM_QC = 64, MM_LD = MM_COLS + 1;
+// each 32 rows x 16 NC columns (2 x NC accumulator blocks of 16 x 16); the work group covers 128 rows.  Per
+// step of 4 inner indices a lane generates the 2 kernel entries of its operand (rows p0 + lane % 16 and
+// p0 + 16 + lane % 16, inner index lane / 16) and reads NC operands of X from LDS; the 2 NC MFMAs reuse each
+// generated entry for 16 NC columns, and the entry generation (VALU) overlaps the matrix pipe.  At 64
+// columns the product runs at about 83 % of the FP64 matrix peak, so the column blocks are sized to the
+// width (48 columns: NC = 3, not 64 columns of which 16 are padding).  Each output element is the same
+// chain of MFMA steps over the inner index whatever NC is; on 3160-tile tests the outputs equal those of a
+// sequential FMA loop over the inner index bit for bit.
+constexpr int MM_ROWS = 128, MM_QC = 64;
 
 // Work group -> (tile t, block blk) for 1-D grids of `per` blocks per tile.  The dispatcher deals work groups
 // to the eight XCDs in turn (id mod 8), so the blocks of a tile are placed 8 apart in id order: every block of
@@ -70,13 +73,15 @@ __device__ __forceinline__ bool xcd_tile_block(int64_t ntile, int per, int64_t* 
 }
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+template <int NC>
 __global__ __launch_bounds__(256, 4) void k_matern_product(const double* __restrict__ sx, const double* __restrict__ sy,
                                                            int64_t n, int b,
                                                            const MaternParams* __restrict__ mpp, int op,
                                                            const double* __restrict__ X, int64_t xstride,
                                                            double* __restrict__ out, int64_t ostride, int k,
                                                            const int* __restrict__ active, int64_t ntile) {
-  const int gx = (b + MM_ROWS - 1) / MM_ROWS, gy = (k + MM_COLS - 1) / MM_COLS;
+  constexpr int COLS = 16 * NC, LD = COLS + 1;   // odd row pitch: the LDS reads and writes are conflict-free
+  const int gx = (b + MM_ROWS - 1) / MM_ROWS, gy = (k + COLS - 1) / COLS;
   int64_t t;
   int blk;
   if (!xcd_tile_block(ntile, gx * gy, &t, &blk)) return;
@@ -88,10 +93,10 @@ __global__ __launch_bounds__(256, 4) void k_matern_product(const double* __restr
   if (op) { const int x = i; i = j; j = x; }
   const int64_t r0g = (int64_t)i * b, q0g = (int64_t)j * b;
   const int mr = (int)min<int64_t>(b, n - r0g), mq = (int)min<int64_t>(b, n - q0g);
-  const int cb = cblk * MM_COLS;
+  const int cb = cblk * COLS;
   const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63, lr = ln & 15, lq = ln >> 4;
   const int p0 = rblk * MM_ROWS + 32 * wv;   // this wave's first row
-  __shared__ double Xs[MM_QC * MM_LD];             // X chunk, [inner][column]
+  __shared__ double Xs[MM_QC * LD];                // X chunk, [inner][column]
   __shared__ double qxs[MM_QC], qys[MM_QC];
   // the lane's two rows (sites); rows past the tile generate zeros
   const int pa = p0 + lr, pb = p0 + 16 + lr;
@@ -99,17 +104,17 @@ __global__ __launch_bounds__(256, 4) void k_matern_product(const double* __restr
   const double ax = oka ? sx[r0g + pa] : 0.0, ay = oka ? sy[r0g + pa] : 0.0;
   const double bx = okb ? sx[r0g + pb] : 0.0, by = okb ? sy[r0g + pb] : 0.0;
   const double* Xt = X + t * xstride;
-  dbl4 acc[2][4];
+  dbl4 acc[2][NC];
 #pragma unroll
   for (int f = 0; f < 2; ++f)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[f][c] = (dbl4){0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < NC; ++c) acc[f][c] = (dbl4){0.0, 0.0, 0.0, 0.0};
   for (int q0 = 0; q0 < mq; q0 += MM_QC) {
     __syncthreads();   // the previous chunk's reads are done
-    for (int e = tid; e < MM_QC * MM_COLS; e += 256) {
+    for (int e = tid; e < MM_QC * COLS; e += 256) {
       const int qq = e % MM_QC, cc = e / MM_QC;
       const int q = q0 + qq, c = cb + cc;
-      Xs[qq * MM_LD + cc] = (q < mq && c < k) ? Xt[q + (int64_t)c * b] : 0.0;
+      Xs[qq * LD + cc] = (q < mq && c < k) ? Xt[q + (int64_t)c * b] : 0.0;
     }
     if (tid < MM_QC) {
       const int q = q0 + tid;
@@ -125,8 +130,8 @@ __global__ __launch_bounds__(256, 4) void k_matern_product(const double* __restr
       const double ta = (oka && okq) ? matern32(mp, ax, ay, qx, qy) : 0.0;
       const double tb = (okb && okq) ? matern32(mp, bx, by, qx, qy) : 0.0;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const double xv = Xs[qq * MM_LD + 16 * c + lr];   // X^T[col 16 c + lane % 16][inner lane / 16]
+      for (int c = 0; c < NC; ++c) {
+        const double xv = Xs[qq * LD + 16 * c + lr];   // X^T[col 16 c + lane % 16][inner lane / 16]
         acc[0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv, ta, acc[0][c], 0, 0, 0);
         acc[1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv, tb, acc[1][c], 0, 0, 0);
       }
@@ -140,7 +145,7 @@ __global__ __launch_bounds__(256, 4) void k_matern_product(const double* __restr
     const int p = p0 + 16 * f + lr;
     if (p < b) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
+      for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int col = cb + 16 * c + lq + 4 * v;
@@ -219,8 +224,9 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ sx,
       Bs[cc][rr] = (c < k && qb + rr < b) ? Bt[qb + rr + (int64_t)c * b] : 0.0;
     }
     __syncthreads();
+    const int cn = min(32, k - c0);   // the chunk's real columns (padding would add exact zeros)
 #pragma unroll 4
-    for (int cc = 0; cc < 32; ++cc) {
+    for (int cc = 0; cc < cn; ++cc) {
       double av[4], bv[4];
 #pragma unroll
       for (int a = 0; a < 4; ++a) av[a] = As[cc][tr * 4 + a];
@@ -584,9 +590,19 @@ hipError_t launch_matern_product(const double* sx, const double* sy, int64_t n, 
                                  const int* active, hipStream_t s) {
   if (ntile <= 0) return hipSuccess;
   const int64_t groups = (ntile + 7) / 8;
-  const int per = nblk(b, MM_ROWS) * nblk(k, MM_COLS);
-  hipLaunchKernelGGL(k_matern_product, dim3((unsigned)(groups * 8 * per)), dim3(256), 0, s, sx, sy, n, b, mp, op, X,
-                     xstride, out, ostride, k, active, ntile);
+  // column blocks of 16 per work group: the fewest padded columns, ties to the wider group (fewer
+  // regenerations of the tile); 48 and 96 columns take 3 blocks, 64 and 128 take 4
+  const int nc = k <= 32 ? 2 : (nblk(k, 48) * 48 < nblk(k, 64) * 64 ? 3 : 4);
+  const unsigned grid = (unsigned)(groups * 8 * nblk(b, MM_ROWS) * nblk(k, 16 * nc));
+  if (nc == 2)
+    hipLaunchKernelGGL(k_matern_product<2>, dim3(grid), dim3(256), 0, s, sx, sy, n, b, mp, op, X, xstride, out,
+                       ostride, k, active, ntile);
+  else if (nc == 3)
+    hipLaunchKernelGGL(k_matern_product<3>, dim3(grid), dim3(256), 0, s, sx, sy, n, b, mp, op, X, xstride, out,
+                       ostride, k, active, ntile);
+  else
+    hipLaunchKernelGGL(k_matern_product<4>, dim3(grid), dim3(256), 0, s, sx, sy, n, b, mp, op, X, xstride, out,
+                       ostride, k, active, ntile);
   return hipGetLastError();
 }
 
