@@ -52,11 +52,11 @@ __global__ void k_gen_diag(const double* __restrict__ sx, const double* __restri
 // each 32 rows x 16 NC columns (2 x NC accumulator blocks of 16 x 16); the work group covers 128 rows.  Per
 // step of 4 inner indices a lane generates the 2 kernel entries of its operand (rows p0 + lane % 16 and
 // p0 + 16 + lane % 16, inner index lane / 16) and reads NC operands of X from LDS; the 2 NC MFMAs reuse each
-// generated entry for 16 NC columns, and the entry generation (VALU) overlaps the matrix pipe.  At 64
-// columns the product runs at about 83 % of the FP64 matrix peak, so the column blocks are sized to the
-// width (48 columns: NC = 3, not 64 columns of which 16 are padding).  Each output element is the same
-// chain of MFMA steps over the inner index whatever NC is; on 3160-tile tests the outputs equal those of a
-// sequential FMA loop over the inner index bit for bit.
+// generated entry for 16 NC columns.  On gfx950 the FP64 VALU work of the entries and the FP64 MFMAs do not
+// overlap (the time per step is close to the sum of the two), so padding columns cost as much as real ones:
+// the column blocks are sized to the width (48 columns: NC = 3, not 64 columns of which 16 are padding).
+// Each output element is the same chain of MFMA steps over the inner index whatever NC is; on 3160-tile
+// tests the outputs equal those of a sequential FMA loop over the inner index bit for bit.
 constexpr int MM_ROWS = 128, MM_QC = 64;
 
 // Work group -> (tile t, block blk) for 1-D grids of `per` blocks per tile.  The dispatcher deals work groups
@@ -188,6 +188,12 @@ __global__ __launch_bounds__(64 * NW) void k_panel_qr(double* __restrict__ M, in
 // For tile t with T ~ A B^T (A: b x k over the row sites of tile i, B: b x k over the sites of tile j),
 // partial sums over 64 x 64 blocks of ||T - A B^T||_F^2 and ||T||_F^2, written to
 // part[(t * nblk + blk) * 2 + {0, 1}].  Deterministic: the rank kernel adds them in block order.
+// A B^T on the matrix cores (v_mfma_f64_16x16x4f64): each wave forms a 32 x 32 quarter of the block as
+// 2 x 2 accumulator tiles, its operands read straight from the tile's panels (16-row runs of 128 bytes; the
+// 64 blocks of a tile run on one XCD and share its L2, so there is no staging and no barrier in the loop);
+// the entries of T are then generated in the accumulators' layout (item v of lane l: row 16 f + lane / 16 +
+// 4 v, column 16 g + lane % 16 of the quarter) and the squared differences summed per lane, then over the
+// wave and over the four waves in a fixed order.
 __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ sx, const double* __restrict__ sy,
                                                   int64_t n, int b, const MaternParams* __restrict__ mpp,
                                                   const double* __restrict__ A, const double* __restrict__ B,
@@ -204,57 +210,59 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ sx,
   const int64_t r0g = (int64_t)i * b, q0g = (int64_t)j * b;
   const int mr = (int)min<int64_t>(b, n - r0g), mq = (int)min<int64_t>(b, n - q0g);
   const int pb = (blk % nbb) * 64, qb = (blk / nbb) * 64;
-  __shared__ double As[32][64];
-  __shared__ double Bs[32][64];
+  __shared__ double ps[2][64], qs[2][64];   // coordinates of the block's row and column sites
   __shared__ double red[2][4];
-  const int tid = threadIdx.x;
-  const int tr = tid / 16, tc = tid % 16;   // 16 x 16 groups of 4 x 4
+  const int tid = threadIdx.x, wv = tid >> 6, ln = tid & 63, lr = ln & 15, lq = ln >> 4;
+  const int pw = (wv & 1) * 32, qw = (wv >> 1) * 32;   // the wave's quarter
+  if (tid < 64) {
+    const int p = pb + tid;
+    ps[0][tid] = p < mr ? sx[r0g + p] : 0.0;
+    ps[1][tid] = p < mr ? sy[r0g + p] : 0.0;
+  } else if (tid < 128) {
+    const int q = qb + tid - 64;
+    qs[0][tid - 64] = q < mq ? sx[q0g + q] : 0.0;
+    qs[1][tid - 64] = q < mq ? sy[q0g + q] : 0.0;
+  }
+  __syncthreads();
   const double* At = A + t * stride;
   const double* Bt = B + t * stride;
-  double acc[4][4];
+  dbl4 acc[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int f = 0; f < 2; ++f)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
-  for (int c0 = 0; c0 < k; c0 += 32) {
-    for (int e = tid; e < 32 * 64; e += 256) {
-      const int rr = e % 64, cc = e / 64;
-      const int c = c0 + cc;
-      As[cc][rr] = (c < k && pb + rr < b) ? At[pb + rr + (int64_t)c * b] : 0.0;
-      Bs[cc][rr] = (c < k && qb + rr < b) ? Bt[qb + rr + (int64_t)c * b] : 0.0;
-    }
-    __syncthreads();
-    const int cn = min(32, k - c0);   // the chunk's real columns (padding would add exact zeros)
+    for (int g = 0; g < 2; ++g) acc[f][g] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  const int pa = pb + pw + lr, qa = qb + qw + lr;   // rows of the lane's operands (and + 16)
+  const bool va0 = pa < b, va1 = pa + 16 < b, vb0 = qa < b, vb1 = qa + 16 < b;
 #pragma unroll 4
-    for (int cc = 0; cc < cn; ++cc) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) av[a] = As[cc][tr * 4 + a];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) bv[c] = Bs[cc][tc * 4 + c];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[a][c] = fma(av[a], bv[c], acc[a][c]);
-    }
-    __syncthreads();
+  for (int s0 = 0; s0 < k; s0 += 4) {
+    const int c = s0 + lq;
+    const bool okc = c < k;   // columns past k add exact zeros
+    const int64_t co = (int64_t)c * b;
+    const double a0 = (okc && va0) ? At[pa + co] : 0.0, a1 = (okc && va1) ? At[pa + 16 + co] : 0.0;
+    const double b0 = (okc && vb0) ? Bt[qa + co] : 0.0, b1 = (okc && vb1) ? Bt[qa + 16 + co] : 0.0;
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
   }
   double res2 = 0.0, fro2 = 0.0;
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int p = pb + tr * 4 + a;
-    if (p >= mr) continue;
-    const double px = sx[r0g + p], py = sy[r0g + p];
+  for (int f = 0; f < 2; ++f)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int q = qb + tc * 4 + c;
-      if (q >= mq) continue;
-      const double tv = matern32(mp, px, py, sx[q0g + q], sy[q0g + q]);
-      const double d = tv - acc[a][c];
-      res2 += d * d;
-      fro2 += tv * tv;
+    for (int g = 0; g < 2; ++g) {
+      const int ql = qw + 16 * g + lr;
+      const bool okq = qb + ql < mq;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int pl = pw + 16 * f + lq + 4 * v;
+        if (okq && pb + pl < mr) {
+          const double tv = matern32(mp, ps[0][pl], ps[1][pl], qs[0][ql], qs[1][ql]);
+          const double d = tv - acc[f][g][v];
+          res2 += d * d;
+          fro2 += tv * tv;
+        }
+      }
     }
-  }
   res2 = wave_sum(res2);
   fro2 = wave_sum(fro2);
   if (lane_id() == 0) { red[0][wave_id()] = res2; red[1][wave_id()] = fro2; }
