@@ -80,7 +80,9 @@ __global__ __launch_bounds__(256, 4) void k_matern_product(const double* __restr
                                                            const double* __restrict__ X, int64_t xstride,
                                                            double* __restrict__ out, int64_t ostride, int k,
                                                            const int* __restrict__ active, int64_t ntile) {
-  constexpr int COLS = 16 * NC, LD = COLS + 1;   // odd row pitch: the LDS reads and writes are conflict-free
+  // odd row pitch: the staging writes (64 consecutive inner indices of one column per wave) are conflict-free;
+  // an operand read spans two rows per half-wave, which share banks at most two ways
+  constexpr int COLS = 16 * NC, LD = COLS + 1;
   const int gx = (b + MM_ROWS - 1) / MM_ROWS, gy = (k + COLS - 1) / COLS;
   int64_t t;
   int blk;
