@@ -1,7 +1,9 @@
 """Edge shapes of the tile grid against the oracle: a short last tile, tile sizes that are not multiples of
-the 32/64-row blocks of the kernels, a two-tile problem, and both rank rules; initial and final ranks
-identical, log L, log-det and quadratic form within 1e-6 * max(1, |v|) (the reference checker's tolerance).
-Also the rank capacity: a theta whose tile needs more than 64 columns fails alone in its batch."""
+the 32/64-row blocks of the kernels, a two-tile problem, tiles narrower than the 96-column second round (b = 60:
+45 of 66 tiles are redone at k = b = 60 columns, ranks up to 49; b = 40: a single round at k = b), and both
+rank rules; initial and final ranks identical, log L, log-det and quadratic form within 1e-6 * max(1, |v|)
+(the reference checker's tolerance).  Also the rank capacity: a theta whose tile needs more than 64 columns
+fails alone in its batch."""
 import os
 
 import numpy as np
@@ -17,6 +19,8 @@ CASES = [  # (N, b, theta, rule, value)
     (777, 128, (1.5, 0.45, 0.08), "fixed", 1e-5),
     (300, 250, (1.0, 0.2, 0.1), "adaptive", 0.5),
     (1500, 500, (2.0, 0.6, 0.01), "fixed", 1e-4),
+    (700, 60, (1.0, 0.25, 0.05), "fixed", 1e-10),
+    (500, 40, (0.8, 0.2, 0.02), "fixed", 1e-6),
 ]
 
 
