@@ -37,7 +37,8 @@ __device__ __forceinline__ double sqrt_dist(double d2) {
 
 // a * b + c with the constant c read from a scalar register pair by the three-operand form: left to the
 // compiler, a Horner step with a constant addend becomes a copy of the constant into a vector register and
-// the two-address v_fmac (two instructions instead of one)
+// the two-address v_fmac (two instructions instead of one).  c must be the same in every lane (a scalar
+// register holds one value per wave): call it with literal constants only.
 __device__ __forceinline__ double fma_sc(double a, double b, double c) {
   double d;
   asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
