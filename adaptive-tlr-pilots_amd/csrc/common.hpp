@@ -17,22 +17,23 @@ struct MaternParams {
 
 // The two elementary functions of an entry, written for the arguments it has (the general library routines
 // spend most of their instructions on range scaling and special classes that never occur here).  Together
-// they take about 38 VALU instructions per entry against about 62, and on gfx950 the FP64 VALU work of the
+// they take about 36 VALU instructions per entry against about 62, and on gfx950 the FP64 VALU work of the
 // entries and the FP64 matrix-core work of the products that consume them do not overlap.
 
 // sqrt(d2) for finite d2 >= 0: the hardware reciprocal square root (relative error about 2^-23), one
 // Goldschmidt step (g = d2 y, h = y / 2 refined together; error squared) and one Newton correction of the
-// root (Markstein's formulation): within 0.5 ulp on the host with y perturbed by up to 2^-22.  0 below
-// 1e-200, where the entry equals its value at distance 0 to the last bit.
+// root (Markstein's formulation): within 0.5 ulp on the host with y perturbed by up to 2^-22.  d2 is raised
+// to at least 1e-300 first (0 and NaN included, as fmax does): a distance below 1e-100 gives an entry equal
+// to its value at distance 0 to the last bit, since e^-x and 1 + x both round to 1 there.
 __device__ __forceinline__ double sqrt_dist(double d2) {
+  d2 = fmax(d2, 1e-300);
   const double y = __builtin_amdgcn_rsq(d2);
   double g = d2 * y, h = 0.5 * y;
   const double r = fma(-g, h, 0.5);
   g = fma(g, r, g);
   h = fma(h, r, h);
   const double e = fma(-g, g, d2);
-  g = fma(e, h, g);
-  return d2 > 1e-200 ? g : 0.0;
+  return fma(e, h, g);
 }
 
 // a * b + c with the constant c read from a scalar register pair by the three-operand form: left to the
