@@ -506,6 +506,35 @@ __global__ void k_copy_int(int* dst, const int* src, int64_t count) {
     dst[e] = src[e];
 }
 
+// Per-slot state at the start of an evaluation, slots 0 .. m-1 in one launch (grid.y = slot): status words
+// zeroed with ST_FAILED = INT_MAX, rhs = y, the arithmetic log = -1 (no recompression run) and, for the mixed
+// rule, every tile's storage precision = 64.  One launch instead of four or five per slot: with 80 slots the
+// per-call enqueue cost of the small fills and copies had left the GPU idle for milliseconds.
+__global__ void k_slot_init(const SlotPtrs* __restrict__ slots, const double* __restrict__ y, int64_t n,
+                            int64_t ntile, int64_t nlog, int mixed) {
+  const SlotPtrs p = slots[blockIdx.y];
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, step = (int64_t)gridDim.x * blockDim.x;
+  if (t0 < ST_COUNT) p.status[t0] = t0 == ST_FAILED ? 0x7fffffff : 0;
+  for (int64_t e = t0; e < n; e += step) p.rhs[e] = y[e];
+  for (int64_t e = t0; e < nlog; e += step) p.rpl[e] = -1;
+  if (mixed)
+    for (int64_t e = t0; e < ntile; e += step) p.prec[e] = 64;
+}
+
+// Replay records of slots 0 .. m-1 in one launch (grid.y = slot s): the initial ranks (ntile) and the rank
+// and arithmetic logs (nlog each) of slot s into record s.
+__global__ void k_slot_records(const SlotPtrs* __restrict__ slots, int* __restrict__ rank0, int* __restrict__ rlog,
+                               int* __restrict__ rpl, int64_t ntile, int64_t nlog) {
+  const int s = blockIdx.y;
+  const SlotPtrs p = slots[s];
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = t0; e < ntile; e += step) rank0[s * ntile + e] = p.rank0[e];
+  for (int64_t e = t0; e < nlog; e += step) {
+    rlog[s * nlog + e] = p.rlog[e];
+    rpl[s * nlog + e] = p.rpl[e];
+  }
+}
+
 // Compact copy of the initially compressed factors of one slot: tile t's first r0[t] columns of U and V
 // (b rows each) live at stash + 2 b off[t] (U) and stash + 2 b off[t] + b r0[t] (V), off = exclusive prefix
 // sum of r0.  dir 0: slot -> stash; dir 1: stash -> slot, and rank[t] = rank0[t] = r0[t].
@@ -584,6 +613,26 @@ hipError_t launch_fill_int(int* p, int64_t count, int value, hipStream_t s) {
 hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_copy_int, dim3(min(nblk(count, 256), 1024)), dim3(256), 0, s, dst, src, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_slot_init(const SlotPtrs* slots, int m, const double* y, int64_t n, int64_t ntile, int64_t nlog,
+                            int mixed, hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  int64_t most = n > nlog ? n : nlog;
+  if (ntile > most) most = ntile;
+  hipLaunchKernelGGL(k_slot_init, dim3(min(nblk(most, 256), 64), (unsigned)m), dim3(256), 0, s, slots, y, n, ntile,
+                     nlog, mixed);
+  return hipGetLastError();
+}
+
+hipError_t launch_slot_records(const SlotPtrs* slots, int m, int* rank0, int* rlog, int* rpl, int64_t ntile,
+                               int64_t nlog, hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  const int64_t most = ntile > nlog ? ntile : nlog;
+  if (most <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_slot_records, dim3(min(nblk(most, 256), 64), (unsigned)m), dim3(256), 0, s, slots, rank0,
+                     rlog, rpl, ntile, nlog);
   return hipGetLastError();
 }
 

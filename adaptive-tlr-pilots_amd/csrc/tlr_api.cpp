@@ -794,14 +794,7 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
   // the two side streams (each with its own workspace), which overlap one slot's latency-bound kernels
   // (panel QR, rank decision) and launch tails with another's.  When a compression kernel is being timed
   // (its events are recorded on the main stream) every slot stays on the main stream.
-  for (int s = 0; s < m; ++s) {
-    const SlotPtrs& p = c->slots[s].p;
-    HIPCHK(launch_fill_int(p.status, ST_COUNT, 0, st));
-    HIPCHK(launch_fill_int(p.status + ST_FAILED, 1, 0x7fffffff, st));
-    HIPCHK(hipMemcpyAsync(p.rhs, c->y, c->n * sizeof(double), hipMemcpyDeviceToDevice, st));
-    if (mixed) HIPCHK(launch_fill_int(p.prec, c->ntile, 64, st));
-    HIPCHK(launch_fill_int(p.rpl, c->nlog, -1, st));
-  }
+  HIPCHK(launch_slot_init(c->d_slots, m, c->y, c->n, c->ntile, c->nlog, mixed, st));
   int ncs = (c->timing_kernel >= 1 && c->timing_kernel <= 3) ? 1 : c->nset;
   if (ncs > m) ncs = m;
   if (ncs > 1) {
@@ -1124,17 +1117,8 @@ int tlr_loglik_grad_batch(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
                      ranks_factor, nullptr, 1);
   if (rc < 0) return rc;
   if (status) memcpy(status, st.data(), sizeof(int32_t) * m);
-  // keep the base ranks as replay records
-  for (int s = 0; s < m; ++s) {
-    const SlotPtrs& p = c->slots[s].p;
-    if (c->ntile > 0)
-      HIPCHK(hipMemcpyAsync(c->rep_rank0 + (int64_t)s * c->ntile, p.rank0, c->ntile * sizeof(int),
-                            hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->rep_rlog + (int64_t)s * c->nlog, p.rlog, c->nlog * sizeof(int), hipMemcpyDeviceToDevice,
-                          c->stream));
-    HIPCHK(hipMemcpyAsync(c->rep_rpl + (int64_t)s * c->nlog, p.rpl, c->nlog * sizeof(int), hipMemcpyDeviceToDevice,
-                          c->stream));
-  }
+  // keep the base ranks as replay records (record s = slot s)
+  HIPCHK(launch_slot_records(c->d_slots, m, c->rep_rank0, c->rep_rlog, c->rep_rpl, c->ntile, c->nlog, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   // d/dlog sigma2 in closed form: tiles and budgets both scale with sigma2, so the ranks are unchanged, L
   // scales by sigma and log L = -(quad / sigma2 + logdet + N log sigma2 + N log 2 pi) / 2 in sigma2

@@ -118,6 +118,12 @@ hipError_t launch_fill_int(int* p, int64_t count, int value, hipStream_t s);
 hipError_t launch_stash(double* U, double* V, int64_t fstride, int b, const int* r0, const int64_t* off,
                         double* stash, int* rank, int* rank0, int64_t ntile, int dir, hipStream_t s);
 hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t s);
+// Start-of-evaluation state of slots 0 .. m-1 (status, rhs = y, arithmetic log, mixed-rule precisions).
+hipError_t launch_slot_init(const SlotPtrs* slots, int m, const double* y, int64_t n, int64_t ntile, int64_t nlog,
+                            int mixed, hipStream_t s);
+// Replay records (initial ranks, rank log, arithmetic log) of slots 0 .. m-1 into records 0 .. m-1.
+hipError_t launch_slot_records(const SlotPtrs* slots, int m, int* rank0, int* rlog, int* rpl, int64_t ntile,
+                               int64_t nlog, hipStream_t s);
 // dense analysis of the rule's tiles: Sigma(theta) (N x N) and Delta = Sigma^TLR - Sigma from the compressed
 // factors, with ||Delta_ij||_F^2 per tile
 hipError_t launch_dense_cov(const double* sx, const double* sy, int64_t n, MaternParams mp, double* K, int64_t ldk,
