@@ -521,6 +521,23 @@ __global__ void k_slot_init(const SlotPtrs* __restrict__ slots, const double* __
     for (int64_t e = t0; e < ntile; e += step) p.prec[e] = 64;
 }
 
+// Results of slots 0 .. m-1 gathered for one copy to the host (grid.y = slot s): the status words, the
+// result (log L, log-det, quadratic form, spare) and, when rk is given, the initial and final ranks of every
+// tile (rk + 2 s ntile and rk + (2 s + 1) ntile).
+__global__ void k_slot_gather(const SlotPtrs* __restrict__ slots, int* __restrict__ st, double* __restrict__ res,
+                              int* __restrict__ rk, int64_t ntile) {
+  const int s = blockIdx.y;
+  const SlotPtrs p = slots[s];
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, step = (int64_t)gridDim.x * blockDim.x;
+  if (t0 < ST_COUNT) st[s * ST_COUNT + t0] = p.status[t0];
+  if (t0 < 4) res[s * 4 + t0] = p.result[t0];
+  if (rk)
+    for (int64_t e = t0; e < ntile; e += step) {
+      rk[2 * s * ntile + e] = p.rank0[e];
+      rk[(2 * s + 1) * ntile + e] = p.rank[e];
+    }
+}
+
 // Replay records of slots 0 .. m-1 in one launch (grid.y = slot s): the initial ranks (ntile) and the rank
 // and arithmetic logs (nlog each) of slot s into record s.
 __global__ void k_slot_records(const SlotPtrs* __restrict__ slots, int* __restrict__ rank0, int* __restrict__ rlog,
@@ -623,6 +640,15 @@ hipError_t launch_slot_init(const SlotPtrs* slots, int m, const double* y, int64
   if (ntile > most) most = ntile;
   hipLaunchKernelGGL(k_slot_init, dim3(min(nblk(most, 256), 64), (unsigned)m), dim3(256), 0, s, slots, y, n, ntile,
                      nlog, mixed);
+  return hipGetLastError();
+}
+
+hipError_t launch_slot_gather(const SlotPtrs* slots, int m, int* st, double* res, int* rk, int64_t ntile,
+                              hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  const int64_t most = rk && ntile > 4 ? ntile : 4;
+  hipLaunchKernelGGL(k_slot_gather, dim3(min(nblk(most, 256), 64), (unsigned)m), dim3(256), 0, s, slots, st, res, rk,
+                     ntile);
   return hipGetLastError();
 }
 

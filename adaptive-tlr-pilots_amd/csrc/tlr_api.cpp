@@ -103,6 +103,9 @@ struct tlr_ctx {
   }
   int64_t factorizations = 0;          // evaluations run (theta factorized), for reports
   SlotPtrs* d_slots = nullptr;
+  int* d_gst = nullptr;                // gathered status words of a batch (ST_COUNT per slot)
+  double* d_gres = nullptr;            // gathered results (4 per slot)
+  int* d_grk = nullptr;                // gathered initial and final ranks (2 ntile per slot)
   int64_t scr_stride = 0;
   double* scr_pool = nullptr;          // recompression scratch, one scr_stride per resident work group
   double* scr_cls[RCLASSES] = {nullptr, nullptr, nullptr, nullptr};   // the part of each class
@@ -439,6 +442,9 @@ static int ctx_build(tlr_ctx* c, const double* S, const double* y, int64_t n, in
   ACHK(c->alloc(&c->d_stash_off, (size_t)max_batch * nt));
   ACHK(c->alloc(&c->d_replay, (size_t)max_batch));
   ACHK(c->alloc(&c->d_slots, (size_t)max_batch));
+  ACHK(c->alloc(&c->d_gst, (size_t)max_batch * ST_COUNT));
+  ACHK(c->alloc(&c->d_gres, (size_t)max_batch * 4));
+  ACHK(c->alloc(&c->d_grk, (size_t)max_batch * 2 * (nt > 0 ? nt : 1)));
   ACHK(c->alloc(&c->d_retry, (size_t)max_batch));
   ACHK(c->alloc(&c->d_thr, (size_t)2 * max_batch));
   ACHK(c->alloc(&c->d_mp, (size_t)max_batch));
@@ -866,7 +872,17 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
     const int rc = enqueue();
     if (rc != TLR_OK) return rc;
   }
+  // every slot's status and result (and ranks, if asked for) in one gather, read back with one copy each
+  // rather than two to four synchronous copies per slot
+  const bool want_ranks = (ranks_sigma || ranks_factor) && c->ntile > 0;
+  HIPCHK(launch_slot_gather(c->d_slots, m, c->d_gst, c->d_gres, want_ranks ? c->d_grk : nullptr, c->ntile, st));
   HIPCHK(hipStreamSynchronize(st));
+  std::vector<int> gst((size_t)m * ST_COUNT);
+  std::vector<double> gres((size_t)m * 4);
+  std::vector<int> grk(want_ranks ? (size_t)m * 2 * c->ntile : 0);
+  HIPCHK(hipMemcpy(gst.data(), c->d_gst, gst.size() * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(gres.data(), c->d_gres, gres.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (want_ranks) HIPCHK(hipMemcpy(grk.data(), c->d_grk, grk.size() * sizeof(int), hipMemcpyDeviceToHost));
   if (!c->timed.empty()) {
     std::vector<int> retry(c->max_batch);
     HIPCHK(hipMemcpy(retry.data(), c->d_retry, c->max_batch * sizeof(int), hipMemcpyDeviceToHost));
@@ -897,13 +913,9 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
     }
   }
   int ret = TLR_OK;
-  std::vector<int> rk;
   for (int s = 0; s < m; ++s) {
-    const SlotPtrs& p = c->slots[s].p;
-    int stv[ST_COUNT];
-    double res[4];
-    HIPCHK(hipMemcpy(stv, p.status, sizeof(stv), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(res, p.result, sizeof(res), hipMemcpyDeviceToHost));
+    const int* stv = gst.data() + (size_t)s * ST_COUNT;
+    const double* res = gres.data() + (size_t)s * 4;
     if (stv[ST_GUARD]) return set_err(TLR_ERR_HIP, "a kernel guard tripped (rank or core out of range)");
     // a tile that needed more columns than the rank capacity: this theta has no result (the others of the
     // batch do); a one-theta call reports it as an error
@@ -932,11 +944,10 @@ static int run_batch_impl(tlr_ctx* c, int32_t m, const double* thetas, int32_t r
     for (int which = 0; which < 2; ++which) {
       int32_t* dst = which == 0 ? ranks_sigma : ranks_factor;
       if (!dst) continue;
-      rk.resize(c->ntile > 0 ? c->ntile : 1);
-      if (c->ntile > 0)
-        HIPCHK(hipMemcpy(rk.data(), which == 0 ? p.rank0 : p.rank, c->ntile * sizeof(int), hipMemcpyDeviceToHost));
       int32_t* d = dst + (int64_t)s * nB * nB;
       memset(d, 0, sizeof(int32_t) * nB * nB);
+      if (!want_ranks) continue;
+      const int* rk = grk.data() + (size_t)(2 * s + which) * c->ntile;
       for (int i = 1; i < nB; ++i)
         for (int j = 0; j < i; ++j) d[i * nB + j] = rk[tri_index(i, j)];
     }

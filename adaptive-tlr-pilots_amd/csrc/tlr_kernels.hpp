@@ -121,6 +121,10 @@ hipError_t launch_copy_int(int* dst, const int* src, int64_t count, hipStream_t 
 // Start-of-evaluation state of slots 0 .. m-1 (status, rhs = y, arithmetic log, mixed-rule precisions).
 hipError_t launch_slot_init(const SlotPtrs* slots, int m, const double* y, int64_t n, int64_t ntile, int64_t nlog,
                             int mixed, hipStream_t s);
+// Status words (ST_COUNT per slot), results (4 per slot) and, if rk, initial and final ranks (2 ntile per slot)
+// of slots 0 .. m-1, for one copy to the host.
+hipError_t launch_slot_gather(const SlotPtrs* slots, int m, int* st, double* res, int* rk, int64_t ntile,
+                              hipStream_t s);
 // Replay records (initial ranks, rank log, arithmetic log) of slots 0 .. m-1 into records 0 .. m-1.
 hipError_t launch_slot_records(const SlotPtrs* slots, int m, int* rank0, int* rlog, int* rpl, int64_t ntile,
                                int64_t nlog, hipStream_t s);
